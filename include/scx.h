@@ -1,0 +1,145 @@
+/*
+ * scx.h -- C ABI of the MI355X (gfx950) contact pipeline for circle (2D) and
+ * sphere (3D) scenes: uniform-grid broad phase, narrow phase, Delassus assembly,
+ * graph-coloured projected Gauss-Seidel Coulomb solve and the symplectic
+ * Euler / Verlet update.
+ *
+ * Plain C: pointers, sizes and int status codes. All array arguments are host
+ * pointers in the reference's layouts unless stated otherwise; the library keeps
+ * its own device-resident copies (one context = one device + one HIP stream).
+ * Every entry point returns SCX_OK (0) or a negative SCX_E* code and sets a
+ * message readable with scx_last_error(). No exception crosses the boundary.
+ * A context is not re-entrant: calls on one context must be serialised by the
+ * caller (as the reference's single-threaded step loop does).
+ *
+ * Reference interfaces replaced (paths relative to the breannansmith/scisim tree):
+ *   scx_potential_overlaps  <- SpatialGridDetector::getPotentialOverlaps
+ *                              (ball2d/SpatialGridDetector.h:39, rigidbody3d/SpatialGridDetector.h:37)
+ *   scx_step                <- ImpactFrictionMap::flow for GeometricImpactFrictionMap
+ *                              (scisim/ConstrainedMaps/ImpactFrictionMap.h:36,
+ *                               GeometricImpactFrictionMap.cpp:267-442), called per step by
+ *                              Ball2DSim::flow (ball2d/Ball2DSim.cpp:305-326) and
+ *                              RigidBody3DSim::flow (rigidbody3d/RigidBody3DSim.cpp:557-630)
+ *   scx_friction_solve      <- FrictionSolver::solve / Sobogus::solve
+ *                              (scisim/ConstrainedMaps/FrictionSolver.h:28, Sobogus.cpp:880-1088)
+ *   scx_unconstrained_flow  <- UnconstrainedMap::flow (scisim/UnconstrainedMaps/UnconstrainedMap.h:33)
+ *                              for SymplecticEulerMap (ball2d/SymplecticEulerMap.cpp:21-32), VerletMap
+ *                              (ball2d/VerletMap.cpp:15-41) and the translational part of SplitHamMap
+ *                              (rigidbody3d/UnconstrainedMaps/SplitHamMap.cpp:17-182)
+ */
+#ifndef SCX_H
+#define SCX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCX_OK 0
+#define SCX_E_ARG (-1)      /* invalid argument */
+#define SCX_E_HIP (-2)      /* HIP runtime error */
+#define SCX_E_CAPACITY (-3) /* a capacity of the context was exceeded */
+#define SCX_E_STATE (-4)    /* call out of order (e.g. step before scx_set_bodies) */
+#define SCX_E_COLOURS (-5)  /* contact graph needs more than 64 colours */
+
+#define SCX_INTEGRATOR_SYMPLECTIC_EULER 0
+#define SCX_INTEGRATOR_VERLET 1
+
+#define SCX_PRECISION_FP64 0       /* every sweep in FP64 */
+#define SCX_PRECISION_FP32_POLISH 1 /* FP32 sweeps, then FP64 polishing sweeps to tolerance */
+
+#define SCX_WARM_NONE 0      /* ImpulsesToCache::NONE */
+#define SCX_WARM_CACHE 1     /* previous-step impulses carried by contact key (ImpulsesToCache::NORMAL_AND_FRICTION) */
+
+typedef struct scx_ctx scx_ctx;
+
+typedef struct scx_config {
+  int32_t dim;          /* 2 (ball2d) or 3 (rigidbody3d spheres) */
+  int32_t device;       /* HIP device ordinal */
+  uint32_t max_bodies;  /* body capacity */
+  uint32_t max_planes;  /* static plane capacity (<= 64) */
+  uint32_t max_contacts; /* contact capacity; 0 = 8 * max_bodies */
+  uint32_t max_pairs;    /* broad-phase pair capacity for scx_potential_overlaps; 0 = 16 * max_bodies */
+} scx_config;
+
+/* Step parameters: the <integrator> and <sobogus_friction_solver> options of the
+ * reference's scene files (ball2dutils/Ball2DSceneParser.cpp:587-640, 1193-1424). */
+typedef struct scx_step_params {
+  double dt;
+  double g[3];          /* gravity; force = m g (ball2d/Forces/Ball2DGravityForce.cpp:36-46) */
+  double mu;            /* friction coefficient, all contacts */
+  double CoR;           /* coefficient of restitution, all contacts */
+  double tol;           /* residual tolerance: max over contacts of ||FB||^2 < tol */
+  uint32_t max_iters;   /* Gauss-Seidel sweep limit */
+  uint32_t eval_every;  /* residual evaluation period in sweeps */
+  int32_t integrator;   /* SCX_INTEGRATOR_* */
+  int32_t precision;    /* SCX_PRECISION_* */
+  int32_t warm_start;   /* SCX_WARM_* */
+  uint32_t iteration;   /* step index (informational) */
+} scx_step_params;
+
+typedef struct scx_step_stats {
+  uint64_t nentries;    /* grid entries (cells covered by all AABBs) */
+  uint32_t npairs;      /* body-body contacts after the exact narrow phase */
+  uint32_t nplane_contacts;
+  uint32_t ncontacts;   /* active set size */
+  uint32_t ncolours;
+  uint32_t sweeps;      /* total GS sweeps (FP32 + FP64 phases) */
+  uint32_t sweeps_fp32;
+  double error;         /* best residual, max ||FB||^2 */
+  int32_t succeeded;    /* error < tol */
+  int32_t pad;
+} scx_step_stats;
+
+const char* scx_version(void);
+const char* scx_last_error(const scx_ctx* ctx);
+
+int scx_create(const scx_config* cfg, scx_ctx** out);
+void scx_destroy(scx_ctx* ctx);
+
+/* SpatialGridDetector::getPotentialOverlaps: all pairs (i<j) of axis-aligned boxes that
+ * overlap (a shared face counts), lexicographically ordered, as pairs_out[2k], pairs_out[2k+1].
+ * aabb_min/aabb_max: n * dim doubles each. Returns SCX_E_CAPACITY with *npairs set to the
+ * required count if cap is too small. */
+int scx_potential_overlaps(scx_ctx* ctx, uint32_t n, const double* aabb_min, const double* aabb_max,
+                           uint64_t cap, uint32_t* pairs_out, uint64_t* npairs);
+
+/* Scene state, in the reference's layouts: q, v = dim*n interleaved (ball2d q, v;
+ * rigidbody3d first 3n entries of q and v); omega = 3n angular velocities (3D; NULL = 0);
+ * r radii; m masses; inertia = isotropic sphere inertia (3D; NULL = 2/5 m r^2). */
+int scx_set_bodies(scx_ctx* ctx, uint32_t n, const double* q, const double* v, const double* omega,
+                   const double* r, const double* m, const double* inertia);
+/* Static planes: point x and normal n (normalised here as StaticPlane does), dim each. */
+int scx_set_planes(scx_ctx* ctx, uint32_t nplanes, const double* x, const double* n);
+int scx_get_state(scx_ctx* ctx, double* q, double* v, double* omega);
+
+/* One GeometricImpactFrictionMap step on the resident state. stats may be NULL. */
+int scx_step(scx_ctx* ctx, const scx_step_params* p, scx_step_stats* stats);
+/* Several steps without host synchronisation between them except where the
+ * pipeline needs counts; stats (may be NULL) receives the last step's stats. */
+int scx_run(scx_ctx* ctx, const scx_step_params* p, uint32_t nsteps, scx_step_stats* stats);
+
+/* Last step's active set in the reference's order (body-body pairs lexicographic, then
+ * plane contacts plane-major): body0, body1 (-1 for planes), plane (-1 for body pairs),
+ * bases (dim*dim per contact, column-major [n s t]), alpha (normal impulse), beta
+ * ((dim-1) per contact), colour. Any output pointer may be NULL. */
+int scx_get_contacts(scx_ctx* ctx, uint32_t cap, int32_t* body0, int32_t* body1, int32_t* plane, double* bases,
+                     double* alpha, double* beta, int32_t* colour, uint32_t* ncontacts);
+/* Last step's post-impact velocity v2 (dim*n; 3D: plus omega2 3n if not NULL). */
+int scx_get_v2(scx_ctx* ctx, double* v2, double* omega2);
+
+/* UnconstrainedMap::flow on host arrays (q0, v0 -> q1, v1), gravity only. */
+int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const double* m, const double* g, double dt,
+                           const double* q0, const double* v0, double* q1, double* v1);
+
+/* Per-phase device time of the last step, from HIP events on the context stream
+ * (enabled with scx_set_timing). names/ms: up to cap entries; returns count. */
+int scx_set_timing(scx_ctx* ctx, int32_t enable);
+int scx_get_timing(scx_ctx* ctx, uint32_t cap, const char** names, double* ms, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCX_H */

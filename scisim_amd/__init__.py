@@ -1,0 +1,233 @@
+"""MI355X (gfx950) contact pipeline for circle/sphere scenes, Python view of the C ABI.
+
+The compute path is libscx.so (hand-written HIP kernels, built in-tree by
+scisim_amd.build). This module only binds the C ABI declared in include/scx.h with
+ctypes; it has no compute fallback: if the library is missing or a call fails, it
+raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libscx.so")
+
+SCX_OK = 0
+SCX_E_ARG = -1
+SCX_E_HIP = -2
+SCX_E_CAPACITY = -3
+SCX_E_STATE = -4
+SCX_E_COLOURS = -5
+
+SYMPLECTIC_EULER = 0
+VERLET = 1
+FP64 = 0
+FP32_POLISH = 1
+WARM_NONE = 0
+WARM_CACHE = 1
+
+# Every symbol include/scx.h declares.
+EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_potential_overlaps", "scx_set_bodies",
+           "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
+           "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing"]
+
+_d = ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+_pu32 = ctypes.POINTER(ctypes.c_uint32)
+_pi32 = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+class ScxConfig(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("device", ctypes.c_int32), ("max_bodies", ctypes.c_uint32),
+                ("max_planes", ctypes.c_uint32), ("max_contacts", ctypes.c_uint32), ("max_pairs", ctypes.c_uint32)]
+
+
+class ScxStepParams(ctypes.Structure):
+    _fields_ = [("dt", _d), ("g", _d * 3), ("mu", _d), ("CoR", _d), ("tol", _d), ("max_iters", ctypes.c_uint32),
+                ("eval_every", ctypes.c_uint32), ("integrator", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("warm_start", ctypes.c_int32), ("iteration", ctypes.c_uint32)]
+
+
+class ScxStepStats(ctypes.Structure):
+    _fields_ = [("nentries", ctypes.c_uint64), ("npairs", ctypes.c_uint32), ("nplane_contacts", ctypes.c_uint32),
+                ("ncontacts", ctypes.c_uint32), ("ncolours", ctypes.c_uint32), ("sweeps", ctypes.c_uint32),
+                ("sweeps_fp32", ctypes.c_uint32), ("error", _d), ("succeeded", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+_lib = None
+
+
+class ScxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libscx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ScxError("libscx.so not found at %s: run `python -m scisim_amd.build` first" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.scx_version.restype = ctypes.c_char_p
+        L.scx_last_error.restype = ctypes.c_char_p
+        L.scx_last_error.argtypes = [_vp]
+        L.scx_create.argtypes = [ctypes.POINTER(ScxConfig), ctypes.POINTER(_vp)]
+        L.scx_destroy.argtypes = [_vp]
+        L.scx_potential_overlaps.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, ctypes.c_uint64, _pu32,
+                                             ctypes.POINTER(ctypes.c_uint64)]
+        L.scx_set_bodies.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, _pd, _pd, _pd, _pd]
+        L.scx_set_planes.argtypes = [_vp, ctypes.c_uint32, _pd, _pd]
+        L.scx_get_state.argtypes = [_vp, _pd, _pd, _pd]
+        L.scx_step.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.POINTER(ScxStepStats)]
+        L.scx_run.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.c_uint32, ctypes.POINTER(ScxStepStats)]
+        L.scx_get_contacts.argtypes = [_vp, ctypes.c_uint32, _pi32, _pi32, _pi32, _pd, _pd, _pd, _pi32, _pu32]
+        L.scx_get_v2.argtypes = [_vp, _pd, _pd]
+        L.scx_unconstrained_flow.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint32, _pd, _pd, _d, _pd, _pd, _pd, _pd]
+        L.scx_set_timing.argtypes = [_vp, ctypes.c_int32]
+        L.scx_get_timing.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p), _pd, _pu32]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_pd):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+
+def step_params(dt, g, mu, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1, integrator=SYMPLECTIC_EULER,
+                precision=FP64, warm_start=WARM_NONE, iteration=1):
+    gg = list(g) + [0.0] * (3 - len(g))
+    return ScxStepParams(dt=dt, g=(_d * 3)(*gg), mu=mu, CoR=CoR, tol=tol, max_iters=max_iters,
+                         eval_every=eval_every, integrator=integrator, precision=precision, warm_start=warm_start,
+                         iteration=iteration)
+
+
+class Context:
+    """One device context (scx_ctx) for 2D circles or 3D spheres."""
+
+    def __init__(self, dim, max_bodies, max_planes=8, device=0, max_contacts=0, max_pairs=0):
+        self.dim = dim
+        self._h = _vp()
+        cfg = ScxConfig(dim=dim, device=device, max_bodies=max_bodies, max_planes=max_planes,
+                        max_contacts=max_contacts, max_pairs=max_pairs)
+        rc = lib().scx_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != SCX_OK:
+            msg = lib().scx_last_error(self._h) if self._h else b"invalid configuration"
+            if self._h:
+                lib().scx_destroy(self._h)
+                self._h = _vp()
+            raise ScxError("scx_create failed (%d): %s" % (rc, msg.decode()))
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().scx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != SCX_OK:
+            raise ScxError("%s failed (%d): %s" % (what, rc, lib().scx_last_error(self._h).decode()))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def potential_overlaps(self, aabb_min, aabb_max):
+        mn, mx = _f64(aabb_min), _f64(aabb_max)
+        n = mn.size // self.dim
+        cap = max(16, 8 * n)
+        np_ = ctypes.c_uint64(0)
+        while True:
+            out = np.empty((cap, 2), dtype=np.uint32)
+            rc = lib().scx_potential_overlaps(self._h, n, _p(mn), _p(mx), cap, _p(out, _pu32), ctypes.byref(np_))
+            if rc == SCX_E_CAPACITY and np_.value > cap:
+                cap = int(np_.value)
+                continue
+            self._check(rc, "scx_potential_overlaps")
+            return out[:np_.value].copy()
+
+    def set_bodies(self, q, v, r, m, omega=None, inertia=None):
+        q, v, r, m = _f64(q), _f64(v), _f64(r), _f64(m)
+        self.n = r.size
+        self._check(lib().scx_set_bodies(self._h, self.n, _p(q), _p(v), _p(_f64(omega)), _p(r), _p(m),
+                                         _p(_f64(inertia))), "scx_set_bodies")
+
+    def set_planes(self, x, n):
+        x, n = _f64(x), _f64(n)
+        self._check(lib().scx_set_planes(self._h, x.size // self.dim, _p(x), _p(n)), "scx_set_planes")
+
+    def get_state(self):
+        q = np.empty(self.dim * self.n)
+        v = np.empty(self.dim * self.n)
+        w = np.empty(3 * self.n) if self.dim == 3 else None
+        self._check(lib().scx_get_state(self._h, _p(q), _p(v), _p(w)), "scx_get_state")
+        return q, v, w
+
+    def step(self, params):
+        st = ScxStepStats()
+        self._check(lib().scx_step(self._h, ctypes.byref(params), ctypes.byref(st)), "scx_step")
+        return st.as_dict()
+
+    def run(self, params, nsteps):
+        st = ScxStepStats()
+        self._check(lib().scx_run(self._h, ctypes.byref(params), nsteps, ctypes.byref(st)), "scx_run")
+        return st.as_dict()
+
+    def contacts(self):
+        D = self.dim
+        nc = ctypes.c_uint32(0)
+        rc = lib().scx_get_contacts(self._h, 0, None, None, None, None, None, None, None, ctypes.byref(nc))
+        C = nc.value
+        if rc != SCX_OK and rc != SCX_E_CAPACITY:
+            self._check(rc, "scx_get_contacts")
+        a = np.empty(C, np.int32)
+        b = np.empty(C, np.int32)
+        pl = np.empty(C, np.int32)
+        basis = np.empty((C, D, D))
+        alpha = np.empty(C)
+        beta = np.empty((C, D - 1))
+        col = np.empty(C, np.int32)
+        self._check(lib().scx_get_contacts(self._h, C, _p(a, _pi32), _p(b, _pi32), _p(pl, _pi32), _p(basis),
+                                           _p(alpha), _p(beta), _p(col, _pi32), ctypes.byref(nc)),
+                    "scx_get_contacts")
+        return {"a": a, "b": b, "plane": pl, "basis": basis, "alpha": alpha, "beta": beta, "colour": col}
+
+    def v2(self):
+        v2 = np.empty(self.dim * self.n)
+        w2 = np.empty(3 * self.n) if self.dim == 3 else None
+        self._check(lib().scx_get_v2(self._h, _p(v2), _p(w2)), "scx_get_v2")
+        return v2, w2
+
+    def unconstrained_flow(self, integrator, m, g, dt, q0, v0):
+        m, q0, v0 = _f64(m), _f64(q0), _f64(v0)
+        gg = _f64(list(g) + [0.0] * (3 - len(g)))
+        q1, v1 = np.empty_like(q0), np.empty_like(v0)
+        self._check(lib().scx_unconstrained_flow(self._h, integrator, m.size, _p(m), _p(gg), dt, _p(q0), _p(v0),
+                                                 _p(q1), _p(v1)), "scx_unconstrained_flow")
+        return q1, v1
+
+    def set_timing(self, on):
+        self._check(lib().scx_set_timing(self._h, 1 if on else 0), "scx_set_timing")
+
+    def timing(self):
+        cnt = ctypes.c_uint32(0)
+        names = (ctypes.c_char_p * 32)()
+        ms = np.zeros(32)
+        self._check(lib().scx_get_timing(self._h, 32, names, _p(ms), ctypes.byref(cnt)), "scx_get_timing")
+        return [(names[i].decode(), float(ms[i])) for i in range(min(cnt.value, 32))]

@@ -1,0 +1,683 @@
+// scx_api.hip -- host side of libscx: context, device buffers, the per-step
+// orchestration of ImpactFrictionMap::flow (GeometricImpactFrictionMap.cpp:267-442)
+// and the C ABI declared in include/scx.h.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "scx_internal.h"
+
+using namespace scx;
+
+namespace {
+
+const char* kVersion = "scx 0.1 (gfx950)";
+
+int fail(scx_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPC(call)                                                                                      \
+  do {                                                                                                  \
+    hipError_t e_ = (call);                                                                             \
+    if (e_ != hipSuccess) return fail(c, SCX_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                              \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, sizeof(T) * count);
+}
+
+template <typename T>
+void dfree(T** p) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+}
+
+uint32_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return (uint32_t)std::min<uint64_t>(p, 1ull << 31);
+}
+
+// Body-sized buffers.
+int alloc_bodies(scx_ctx* c, uint32_t N) {
+  const int D = c->dim;
+  DeviceBuffers& d = c->d;
+  HIPC(dalloc(&d.q, (size_t)D * N));
+  HIPC(dalloc(&d.v, (size_t)D * N));
+  HIPC(dalloc(&d.w, 3 * (size_t)N));
+  HIPC(dalloc(&d.r, N));
+  HIPC(dalloc(&d.m, N));
+  HIPC(dalloc(&d.inertia, N));
+  HIPC(dalloc(&d.q0, (size_t)D * N));
+  HIPC(dalloc(&d.v0, (size_t)D * N));
+  HIPC(dalloc(&d.w0, 3 * (size_t)N));
+  HIPC(dalloc(&d.q1p, (size_t)D * N));
+  HIPC(dalloc(&d.v2, (size_t)D * N));
+  HIPC(dalloc(&d.w2, 3 * (size_t)N));
+  HIPC(dalloc(&d.bv64, 4 * (size_t)N));
+  HIPC(dalloc(&d.bw64, 4 * (size_t)N));
+  HIPC(dalloc(&d.bv32, 4 * (size_t)N));
+  HIPC(dalloc(&d.bw32, 4 * (size_t)N));
+  HIPC(dalloc(&d.aabb_min, (size_t)D * N));
+  HIPC(dalloc(&d.aabb_max, (size_t)D * N));
+  HIPC(dalloc(&d.red, 9 * (size_t)((N + kBlock - 1) / kBlock + 1)));
+  HIPC(dalloc(&d.pcount, (size_t)N + 1));
+  HIPC(dalloc(&d.pmask, std::max<size_t>(N, c->cap_contacts)));
+  HIPC(dalloc(&d.deg, 2 * ((size_t)N + 1)));
+  c->cap_bodies = N;
+  return SCX_OK;
+}
+
+int alloc_entries(scx_ctx* c, uint32_t cap_entries) {
+  DeviceBuffers& d = c->d;
+  c->cap_entries = cap_entries;
+  c->table_size = next_pow2(std::max<uint64_t>(1024, cap_entries));
+  HIPC(dalloc(&d.ekey, cap_entries));
+  HIPC(dalloc(&d.eid, cap_entries));
+  HIPC(dalloc(&d.bucket, 2 * (size_t)c->table_size + 1));
+  return SCX_OK;
+}
+
+template <typename Real>
+int alloc_soa(scx_ctx* c, ContactSoA<Real>& s, size_t C) {
+  HIPC(dalloc(&s.a, C));
+  HIPC(dalloc(&s.b, C));
+  HIPC(dalloc(&s.n, 3 * C));
+  HIPC(dalloc(&s.s, 3 * C));
+  HIPC(dalloc(&s.la, C));
+  HIPC(dalloc(&s.lb, C));
+  HIPC(dalloc(&s.wn, C));
+  HIPC(dalloc(&s.wt, C));
+  HIPC(dalloc(&s.mu, C));
+  HIPC(dalloc(&s.forcing, C));
+  HIPC(dalloc(&s.r, 3 * C));
+  return SCX_OK;
+}
+
+template <typename Real>
+void free_soa(ContactSoA<Real>& s) {
+  dfree(&s.a);
+  dfree(&s.b);
+  dfree(&s.n);
+  dfree(&s.s);
+  dfree(&s.la);
+  dfree(&s.lb);
+  dfree(&s.wn);
+  dfree(&s.wt);
+  dfree(&s.mu);
+  dfree(&s.forcing);
+  dfree(&s.r);
+}
+
+int alloc_contacts(scx_ctx* c, uint32_t C) {
+  const int D = c->dim;
+  DeviceBuffers& d = c->d;
+  c->cap_contacts = C;
+  HIPC(dalloc(&d.ca, C));
+  HIPC(dalloc(&d.cb, C));
+  HIPC(dalloc(&d.cplane, C));
+  HIPC(dalloc(&d.basis, (size_t)D * D * C));
+  HIPC(dalloc(&d.cla, C));
+  HIPC(dalloc(&d.clb, C));
+  HIPC(dalloc(&d.cwn, C));
+  HIPC(dalloc(&d.cwt, C));
+  HIPC(dalloc(&d.cforcing, C));
+  HIPC(dalloc(&d.calpha, C));
+  HIPC(dalloc(&d.cbeta, (size_t)(D - 1) * C));
+  HIPC(dalloc(&d.crinit, (size_t)D * C));
+  HIPC(dalloc(&d.adj, 2 * (size_t)C));
+  HIPC(dalloc(&d.cstate, C));
+  c->cap_perm = std::max<uint32_t>(C, c->cap_bodies);
+  HIPC(dalloc(&d.perm, c->cap_perm));
+  HIPC(dalloc(&d.rbest, 3 * (size_t)C));
+  if (alloc_soa(c, d.c64, C) != SCX_OK) return SCX_E_HIP;
+  if (alloc_soa(c, d.c32, C) != SCX_OK) return SCX_E_HIP;
+  HIPC(dalloc(&d.pmask, std::max<size_t>(c->cap_bodies, C)));
+  const size_t nblk = (std::max<size_t>(C, c->cap_bodies) + kBlock - 1) / kBlock;
+  HIPC(dalloc(&d.mbcount, (size_t)kMaxColours * nblk + 2));
+  return SCX_OK;
+}
+
+int alloc_scan_tmp(scx_ctx* c) {
+  const size_t big = std::max<size_t>({(size_t)c->table_size, (size_t)c->cap_bodies + 1, (size_t)c->cap_pairs,
+                                       (size_t)kMaxColours * ((std::max<size_t>(c->cap_contacts, c->cap_bodies) +
+                                                               kBlock - 1) / kBlock) + 2});
+  HIPC(dalloc(&c->d.scan_tmp, scan_tmp_words((uint32_t)std::min<size_t>(big, 0xFFFFFFFFu)) + 4));
+  return SCX_OK;
+}
+
+void free_all(scx_ctx* c) {
+  DeviceBuffers& d = c->d;
+  double* dbl[] = {d.q, d.v, d.w, d.r, d.m, d.inertia, d.q0, d.v0, d.w0, d.q1p, d.v2, d.w2, d.bv64, d.bw64,
+                   d.px, d.pn, d.aabb_min, d.aabb_max, d.red, d.basis, d.cla, d.clb, d.cwn, d.cwt, d.cforcing,
+                   d.calpha, d.cbeta, d.crinit, d.rbest, d.partial, d.prev_r};
+  for (double* p : dbl)
+    if (p) hipFree(p);
+  float* flt[] = {d.bv32, d.bw32};
+  for (float* p : flt)
+    if (p) hipFree(p);
+  uint32_t* u32[] = {d.bucket, d.eid, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
+                     d.perm, d.coloff};
+  for (uint32_t* p : u32)
+    if (p) hipFree(p);
+  if (d.cplane) hipFree(d.cplane);
+  if (d.cstate) hipFree(d.cstate);
+  if (d.ekey) hipFree(d.ekey);
+  if (d.pmask) hipFree(d.pmask);
+  if (d.prev_key) hipFree(d.prev_key);
+  if (d.cur_key) hipFree(d.cur_key);
+  if (d.grid) hipFree(d.grid);
+  if (d.ctr) hipFree(d.ctr);
+  if (d.ctr_host) hipHostFree(d.ctr_host);
+  if (d.host_u32) hipHostFree(d.host_u32);
+  free_soa(d.c64);
+  free_soa(d.c32);
+  d = DeviceBuffers{};
+}
+
+void timing_mark(scx_ctx* c, const char* name) {
+  if (!c->timing) return;
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  hipEventRecord(ev, c->stream);
+  c->events.push_back(ev);
+  c->event_names.push_back(name);
+}
+
+void timing_collect(scx_ctx* c) {
+  if (!c->timing || c->events.size() < 2) {
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
+    c->events.clear();
+    c->event_names.clear();
+    return;
+  }
+  hipEventSynchronize(c->events.back());
+  c->phase_ms.clear();
+  c->phase_names.clear();
+  for (size_t k = 1; k < c->events.size(); ++k) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->events[k - 1], c->events[k]);
+    c->phase_ms.push_back(ms);
+    c->phase_names.push_back(c->event_names[k]);
+  }
+  for (hipEvent_t e : c->events) hipEventDestroy(e);
+  c->events.clear();
+  c->event_names.clear();
+}
+
+// One geometric-map step. Returns SCX_OK, or SCX_E_CAPACITY after growing a buffer
+// (the caller retries the step from the untouched start-of-step state).
+int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool want_stats) {
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  const uint32_t n = c->n;
+  const int D = c->dim;
+  if (c->events.empty()) timing_mark(c, "start");
+  HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+  // start-of-step state q0, v0, omega0 (the resident state becomes the step input)
+  std::swap(d.q, d.q0);
+  std::swap(d.v, d.v0);
+  std::swap(d.w, d.w0);
+  // predictor + boxes (GeometricImpactFrictionMap.cpp:279)
+  HIPC(launch_predict_aabb(c, d.q0, d.v0, d.m, p->dt, p->g, p->integrator, d.q1p));
+  timing_mark(c, "predict");
+  // broad phase binning
+  HIPC(launch_grid_build(c, n));
+  timing_mark(c, "grid");
+  // body-body active set (Ball2DSim.cpp:553-608 / RigidBody3DSim.cpp:1072-1260)
+  const int filter = (D == 2) ? 1 : 2;
+  HIPC(launch_pairs(c, n, 0, filter, d.ca, d.cb, c->cap_contacts));
+  HIPC(launch_exclusive_scan(d.pcount, d.pcount, n, d.scan_tmp, s));
+  HIPC(launch_pairs(c, n, 1, filter, d.ca, d.cb, c->cap_contacts));
+  timing_mark(c, "pairs");
+  // plane contacts, plane-major (Ball2DSim.cpp:747-764 / RigidBody3DSim.cpp:1414-1502)
+  uint32_t npl = 0;
+  if (c->nplanes > 0) {
+    HIPC(launch_plane_masks(c, n));
+    HIPC(launch_multibucket(c, d.pmask, n, c->nplanes, d.perm, c->cap_perm, d.coloff));
+  }
+  timing_mark(c, "planes");
+  // sync point: sizes of the active set
+  HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(d.host_u32, d.pcount + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (c->nplanes > 0)
+    HIPC(hipMemcpyAsync(d.host_u32 + 1, d.coloff + c->nplanes, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  const uint32_t nbb = d.host_u32[0];
+  if (c->nplanes > 0) npl = d.host_u32[1];
+  const uint64_t nentries = d.ctr_host->nentries;
+  if ((d.ctr_host->overflow & 1u) || nentries > c->cap_entries) {
+    std::swap(d.q, d.q0);
+    std::swap(d.v, d.v0);
+    std::swap(d.w, d.w0);
+    const uint32_t want = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, nentries + nentries / 4 + 1024);
+    if (alloc_entries(c, want) != SCX_OK || alloc_scan_tmp(c) != SCX_OK) return SCX_E_HIP;
+    return SCX_E_CAPACITY;
+  }
+  const uint64_t C64 = (uint64_t)nbb + npl;
+  if ((d.ctr_host->overflow & 6u) || C64 > c->cap_contacts || npl > c->cap_perm) {
+    std::swap(d.q, d.q0);
+    std::swap(d.v, d.v0);
+    std::swap(d.w, d.w0);
+    if (C64 > 0x7FFFFFFFull) return fail(c, SCX_E_CAPACITY, "active set too large (%llu)", (unsigned long long)C64);
+    if (alloc_contacts(c, (uint32_t)(C64 + C64 / 4 + 1024)) != SCX_OK || alloc_scan_tmp(c) != SCX_OK)
+      return SCX_E_HIP;
+    return SCX_E_CAPACITY;
+  }
+  const uint32_t C = (uint32_t)C64;
+  if (C > 0) {
+    HIPC(launch_contact_setup(c, nbb, npl, p->CoR, p->mu));
+    timing_mark(c, "setup");
+    HIPC(launch_colouring(c, C));
+    HIPC(launch_colour_sort(c, C));
+    timing_mark(c, "colour");
+    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr));
+    timing_mark(c, "gather");
+  }
+  HIPC(launch_solve(c, C, p));
+  timing_mark(c, "solve");
+  HIPC(launch_finish(c, C, p));
+  timing_mark(c, "finish");
+  c->last_ncontacts = C;
+  c->last_npairs = nbb;
+  c->last_nplane = npl;
+  if (want_stats || c->timing) {
+    HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (d.ctr_host->overflow & 8u) return fail(c, SCX_E_COLOURS, "contact graph needs more than %d colours", kMaxColours);
+    c->last_ncolours = d.ctr_host->ncolours;
+    if (st) {
+      st->nentries = nentries;
+      st->npairs = nbb;
+      st->nplane_contacts = npl;
+      st->ncontacts = C;
+      st->ncolours = d.ctr_host->ncolours;
+      st->sweeps = d.ctr_host->sweeps;
+      st->sweeps_fp32 = d.ctr_host->sweeps_fp32;
+      st->error = (C == 0) ? 0.0 : d.ctr_host->error;
+      st->succeeded = (C == 0) ? 1 : (d.ctr_host->error < p->tol ? 1 : 0);
+    }
+  }
+  timing_collect(c);
+  return SCX_OK;
+}
+
+int check_params(scx_ctx* c, const scx_step_params* p) {
+  if (!p) return fail(c, SCX_E_ARG, "null step parameters");
+  if (!c->has_bodies) return fail(c, SCX_E_STATE, "scx_set_bodies must be called before stepping");
+  if (!(p->dt > 0.0)) return fail(c, SCX_E_ARG, "dt must be positive");
+  if (!(p->mu >= 0.0)) return fail(c, SCX_E_ARG, "mu must be non-negative");
+  if (!(p->CoR >= 0.0 && p->CoR <= 1.0)) return fail(c, SCX_E_ARG, "CoR must lie in [0, 1]");
+  if (!(p->tol >= 0.0)) return fail(c, SCX_E_ARG, "tol must be non-negative");
+  if (p->integrator != SCX_INTEGRATOR_SYMPLECTIC_EULER && p->integrator != SCX_INTEGRATOR_VERLET)
+    return fail(c, SCX_E_ARG, "unknown integrator %d", p->integrator);
+  if (p->precision != SCX_PRECISION_FP64 && p->precision != SCX_PRECISION_FP32_POLISH)
+    return fail(c, SCX_E_ARG, "unknown precision mode %d", p->precision);
+  if (p->warm_start != SCX_WARM_NONE) return fail(c, SCX_E_ARG, "warm start mode %d is not available", p->warm_start);
+  return SCX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* scx_version(void) { return kVersion; }
+
+const char* scx_last_error(const scx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int scx_create(const scx_config* cfg, scx_ctx** out) {
+  if (!out) return SCX_E_ARG;
+  *out = nullptr;
+  if (!cfg || (cfg->dim != 2 && cfg->dim != 3) || cfg->max_bodies == 0 || cfg->max_planes > 64) return SCX_E_ARG;
+  scx_ctx* c = new scx_ctx;
+  c->dim = cfg->dim;
+  c->device = cfg->device;
+  int rc = SCX_OK;
+  do {
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipSetDevice(%d): %s", cfg->device, hipGetErrorString(e));
+      break;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipGetDeviceProperties failed");
+      break;
+    }
+    c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipStreamCreate failed");
+      break;
+    }
+    const uint32_t N = cfg->max_bodies;
+    c->cap_contacts = cfg->max_contacts ? cfg->max_contacts : 8u * N;
+    c->cap_pairs = cfg->max_pairs ? cfg->max_pairs : 16u * N;
+    c->cap_planes = std::max<uint32_t>(1, cfg->max_planes);
+    if ((rc = alloc_bodies(c, N)) != SCX_OK) break;
+    if ((rc = alloc_entries(c, (c->dim == 2 ? 8u : 16u) * N + 1024)) != SCX_OK) break;
+    if ((rc = alloc_contacts(c, c->cap_contacts)) != SCX_OK) break;
+    if ((rc = alloc_scan_tmp(c)) != SCX_OK) break;
+    DeviceBuffers& d = c->d;
+    if ((e = dalloc(&d.px, 3 * (size_t)c->cap_planes)) != hipSuccess ||
+        (e = dalloc(&d.pn, 3 * (size_t)c->cap_planes)) != hipSuccess ||
+        (e = dalloc(&d.coloff, (size_t)kMaxColours + 2)) != hipSuccess ||
+        (e = dalloc(&d.grid, 1)) != hipSuccess || (e = dalloc(&d.ctr, 1)) != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
+      break;
+    }
+    if (hipHostMalloc((void**)&d.ctr_host, sizeof(StepCounters)) != hipSuccess ||
+        hipHostMalloc((void**)&d.host_u32, 64 * sizeof(uint32_t)) != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipHostMalloc failed");
+      break;
+    }
+    c->gs_blocks = solver_max_blocks(c);
+    c->colour_blocks = colour_max_blocks(c);
+    if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
+      break;
+    }
+  } while (0);
+  if (rc != SCX_OK) {
+    // keep the context so that scx_last_error can report; caller must destroy it
+    *out = c;
+    return rc;
+  }
+  *out = c;
+  return SCX_OK;
+}
+
+void scx_destroy(scx_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_all(c);
+  for (hipEvent_t e : c->events) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int scx_set_bodies(scx_ctx* c, uint32_t n, const double* q, const double* v, const double* omega, const double* r,
+                   const double* m, const double* inertia) {
+  if (!c) return SCX_E_ARG;
+  if (n == 0 || !q || !v || !r || !m) return fail(c, SCX_E_ARG, "scx_set_bodies: null array or n == 0");
+  HIPC(hipSetDevice(c->device));
+  const int D = c->dim;
+  for (uint32_t i = 0; i < n; ++i)
+    if (!(r[i] > 0.0) || !(m[i] > 0.0)) return fail(c, SCX_E_ARG, "body %u: radius and mass must be positive", i);
+  if (n > c->cap_bodies) {
+    if (alloc_bodies(c, n) != SCX_OK) return SCX_E_HIP;
+    if (alloc_entries(c, (D == 2 ? 8u : 16u) * n + 1024) != SCX_OK) return SCX_E_HIP;
+    if (alloc_contacts(c, std::max(c->cap_contacts, 8u * n)) != SCX_OK) return SCX_E_HIP;
+    if (alloc_scan_tmp(c) != SCX_OK) return SCX_E_HIP;
+  }
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  HIPC(hipMemcpyAsync(d.q, q, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.v, v, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.r, r, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.m, m, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  std::vector<double> tmp;
+  if (D == 3) {
+    if (omega)
+      HIPC(hipMemcpyAsync(d.w, omega, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    else
+      HIPC(hipMemsetAsync(d.w, 0, sizeof(double) * 3 * n, s));
+    tmp.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      if (inertia) {
+        tmp[i] = inertia[i];
+      } else {
+        // RigidBodySphere::computeMassAndInertia (rigidbody3d/Geometry/RigidBodySphere.cpp:10-18)
+        tmp[i] = 2.0 * m[i] * r[i] * r[i] / 5.0;
+      }
+      if (!(tmp[i] > 0.0)) return fail(c, SCX_E_ARG, "body %u: inertia must be positive", i);
+    }
+    HIPC(hipMemcpyAsync(d.inertia, tmp.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+  } else {
+    HIPC(hipMemsetAsync(d.w, 0, sizeof(double) * 3 * n, s));
+    HIPC(hipMemsetAsync(d.inertia, 0, sizeof(double) * n, s));
+  }
+  HIPC(hipStreamSynchronize(s));
+  c->n = n;
+  c->has_bodies = true;
+  c->last_ncontacts = 0;
+  return SCX_OK;
+}
+
+int scx_set_planes(scx_ctx* c, uint32_t nplanes, const double* x, const double* nrm) {
+  if (!c) return SCX_E_ARG;
+  if (nplanes > 64) return fail(c, SCX_E_ARG, "at most 64 static planes");
+  if (nplanes > 0 && (!x || !nrm)) return fail(c, SCX_E_ARG, "null plane arrays");
+  HIPC(hipSetDevice(c->device));
+  const int D = c->dim;
+  if (nplanes > c->cap_planes) {
+    HIPC(dalloc(&c->d.px, 3 * (size_t)nplanes));
+    HIPC(dalloc(&c->d.pn, 3 * (size_t)nplanes));
+    c->cap_planes = nplanes;
+  }
+  std::vector<double> nn((size_t)D * nplanes);
+  for (uint32_t p = 0; p < nplanes; ++p) {
+    // StaticPlane normalises its normal (ball2d/StaticGeometry/StaticPlane.cpp:10-20)
+    double z = nrm[(size_t)D * p] * nrm[(size_t)D * p];
+    for (int a = 1; a < D; ++a) z = z + nrm[(size_t)D * p + a] * nrm[(size_t)D * p + a];
+    if (!(z > 0.0)) return fail(c, SCX_E_ARG, "plane %u has a zero normal", p);
+    const double sq = std::sqrt(z);
+    for (int a = 0; a < D; ++a) nn[(size_t)D * p + a] = nrm[(size_t)D * p + a] / sq;
+  }
+  if (nplanes > 0) {
+    HIPC(hipMemcpyAsync(c->d.px, x, sizeof(double) * D * nplanes, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->d.pn, nn.data(), sizeof(double) * D * nplanes, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPC(hipStreamSynchronize(c->stream));
+  c->nplanes = nplanes;
+  return SCX_OK;
+}
+
+int scx_get_state(scx_ctx* c, double* q, double* v, double* omega) {
+  if (!c) return SCX_E_ARG;
+  if (!c->has_bodies) return fail(c, SCX_E_STATE, "no bodies");
+  HIPC(hipSetDevice(c->device));
+  const int D = c->dim;
+  if (q) HIPC(hipMemcpyAsync(q, c->d.q, sizeof(double) * D * c->n, hipMemcpyDeviceToHost, c->stream));
+  if (v) HIPC(hipMemcpyAsync(v, c->d.v, sizeof(double) * D * c->n, hipMemcpyDeviceToHost, c->stream));
+  if (omega && D == 3)
+    HIPC(hipMemcpyAsync(omega, c->d.w, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SCX_OK;
+}
+
+int scx_step(scx_ctx* c, const scx_step_params* p, scx_step_stats* stats) {
+  if (!c) return SCX_E_ARG;
+  int rc = check_params(c, p);
+  if (rc != SCX_OK) return rc;
+  HIPC(hipSetDevice(c->device));
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    rc = step_once(c, p, stats, true);
+    if (rc != SCX_E_CAPACITY) return rc;
+  }
+  return fail(c, SCX_E_CAPACITY, "buffers kept overflowing");
+}
+
+int scx_run(scx_ctx* c, const scx_step_params* p, uint32_t nsteps, scx_step_stats* stats) {
+  if (!c) return SCX_E_ARG;
+  int rc = check_params(c, p);
+  if (rc != SCX_OK) return rc;
+  HIPC(hipSetDevice(c->device));
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    scx_step_params q = *p;
+    q.iteration = p->iteration + k;
+    const bool last = (k + 1 == nsteps);
+    int attempt = 0;
+    for (; attempt < 8; ++attempt) {
+      rc = step_once(c, &q, last ? stats : nullptr, last);
+      if (rc != SCX_E_CAPACITY) break;
+    }
+    if (rc != SCX_OK) return attempt >= 8 ? fail(c, SCX_E_CAPACITY, "buffers kept overflowing") : rc;
+  }
+  return SCX_OK;
+}
+
+int scx_get_contacts(scx_ctx* c, uint32_t cap, int32_t* body0, int32_t* body1, int32_t* plane, double* bases,
+                     double* alpha, double* beta, int32_t* colour, uint32_t* ncontacts) {
+  if (!c) return SCX_E_ARG;
+  HIPC(hipSetDevice(c->device));
+  const uint32_t C = c->last_ncontacts;
+  if (ncontacts) *ncontacts = C;
+  if (C > cap) return fail(c, SCX_E_CAPACITY, "need room for %u contacts", C);
+  const int D = c->dim;
+  hipStream_t s = c->stream;
+  if (C == 0) return SCX_OK;
+  std::vector<uint32_t> tmp(C);
+  if (body0) {
+    HIPC(hipMemcpy(tmp.data(), c->d.ca, sizeof(uint32_t) * C, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < C; ++k) body0[k] = (int32_t)tmp[k];
+  }
+  if (body1) {
+    HIPC(hipMemcpy(tmp.data(), c->d.cb, sizeof(uint32_t) * C, hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < C; ++k) body1[k] = (tmp[k] == kNoBody) ? -1 : (int32_t)tmp[k];
+  }
+  if (plane) HIPC(hipMemcpyAsync(plane, c->d.cplane, sizeof(int32_t) * C, hipMemcpyDeviceToHost, s));
+  if (bases) HIPC(hipMemcpyAsync(bases, c->d.basis, sizeof(double) * D * D * C, hipMemcpyDeviceToHost, s));
+  if (alpha) HIPC(hipMemcpyAsync(alpha, c->d.calpha, sizeof(double) * C, hipMemcpyDeviceToHost, s));
+  if (beta) HIPC(hipMemcpyAsync(beta, c->d.cbeta, sizeof(double) * (D - 1) * C, hipMemcpyDeviceToHost, s));
+  if (colour) {
+    HIPC(hipMemcpyAsync(colour, c->d.cstate, sizeof(int32_t) * C, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    for (uint32_t k = 0; k < C; ++k) colour[k] &= 0xff;
+  }
+  HIPC(hipStreamSynchronize(s));
+  return SCX_OK;
+}
+
+int scx_get_v2(scx_ctx* c, double* v2, double* omega2) {
+  if (!c) return SCX_E_ARG;
+  HIPC(hipSetDevice(c->device));
+  if (v2) HIPC(hipMemcpy(v2, c->d.v2, sizeof(double) * c->dim * c->n, hipMemcpyDeviceToHost));
+  if (omega2 && c->dim == 3) HIPC(hipMemcpy(omega2, c->d.w2, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost));
+  return SCX_OK;
+}
+
+int scx_potential_overlaps(scx_ctx* c, uint32_t n, const double* aabb_min, const double* aabb_max, uint64_t cap,
+                           uint32_t* pairs_out, uint64_t* npairs) {
+  if (!c) return SCX_E_ARG;
+  if (npairs) *npairs = 0;
+  if (n == 0) return SCX_OK;
+  if (!aabb_min || !aabb_max) return fail(c, SCX_E_ARG, "null box arrays");
+  HIPC(hipSetDevice(c->device));
+  const int D = c->dim;
+  if (n > c->cap_bodies) {
+    if (alloc_bodies(c, n) != SCX_OK || alloc_contacts(c, std::max(c->cap_contacts, 8u * n)) != SCX_OK ||
+        alloc_scan_tmp(c) != SCX_OK)
+      return SCX_E_HIP;
+    c->has_bodies = false;
+  }
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  HIPC(hipMemcpyAsync(d.aabb_min, aabb_min, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.aabb_max, aabb_max, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+    HIPC(launch_grid_build(c, n));
+    HIPC(launch_pairs(c, n, 0, 0, nullptr, nullptr, 0));
+    HIPC(launch_exclusive_scan(d.pcount, d.pcount, n, d.scan_tmp, s));
+    HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(d.host_u32, d.pcount + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    const uint64_t ne = d.ctr_host->nentries;
+    if ((d.ctr_host->overflow & 1u) || ne > c->cap_entries) {
+      if (alloc_entries(c, (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, ne + ne / 4 + 1024)) != SCX_OK ||
+          alloc_scan_tmp(c) != SCX_OK)
+        return SCX_E_HIP;
+      continue;
+    }
+    const uint32_t P = d.host_u32[0];
+    if (npairs) *npairs = P;
+    if (P > cap) return fail(c, SCX_E_CAPACITY, "need room for %u pairs", P);
+    if (P == 0) return SCX_OK;
+    if (!d.pa || P > c->cap_pairs) {
+      c->cap_pairs = std::max<uint32_t>(P, c->cap_pairs);
+      HIPC(dalloc(&d.pa, c->cap_pairs));
+      HIPC(dalloc(&d.pb, c->cap_pairs));
+    }
+    HIPC(launch_pairs(c, n, 1, 0, d.pa, d.pb, c->cap_pairs));
+    std::vector<uint32_t> a(P), b(P);
+    HIPC(hipMemcpyAsync(a.data(), d.pa, sizeof(uint32_t) * P, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(b.data(), d.pb, sizeof(uint32_t) * P, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (d.ctr_host->overflow & 2u) return fail(c, SCX_E_CAPACITY, "pair buffer overflow");
+    for (uint32_t k = 0; k < P; ++k) {
+      pairs_out[2 * (size_t)k] = a[k];
+      pairs_out[2 * (size_t)k + 1] = b[k];
+    }
+    return SCX_OK;
+  }
+  return fail(c, SCX_E_CAPACITY, "grid entry buffer kept overflowing");
+}
+
+int scx_unconstrained_flow(scx_ctx* c, int32_t integrator, uint32_t n, const double* m, const double* g, double dt,
+                           const double* q0, const double* v0, double* q1, double* v1) {
+  if (!c) return SCX_E_ARG;
+  if (n == 0) return SCX_OK;
+  if (!m || !g || !q0 || !v0 || !q1 || !v1) return fail(c, SCX_E_ARG, "null array");
+  if (integrator != SCX_INTEGRATOR_SYMPLECTIC_EULER && integrator != SCX_INTEGRATOR_VERLET)
+    return fail(c, SCX_E_ARG, "unknown integrator %d", integrator);
+  HIPC(hipSetDevice(c->device));
+  if (n > c->cap_bodies) {
+    if (alloc_bodies(c, n) != SCX_OK || alloc_contacts(c, std::max(c->cap_contacts, 8u * n)) != SCX_OK ||
+        alloc_scan_tmp(c) != SCX_OK)
+      return SCX_E_HIP;
+    c->has_bodies = false;
+  }
+  const int D = c->dim;
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  // stage through the step scratch buffers (q0/v0 hold the start state, q1p/v2 the result)
+  HIPC(hipMemcpyAsync(d.q1p, q0, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.v2, v0, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.aabb_min, m, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  double gg[3] = {g[0], g[1], D == 3 ? g[2] : 0.0};
+  HIPC(launch_flow(c, n, integrator, d.aabb_min, gg, dt, d.q1p, d.v2, d.q0, d.aabb_max));
+  HIPC(hipMemcpyAsync(q1, d.q0, sizeof(double) * D * n, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(v1, d.aabb_max, sizeof(double) * D * n, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  return SCX_OK;
+}
+
+int scx_set_timing(scx_ctx* c, int32_t enable) {
+  if (!c) return SCX_E_ARG;
+  c->timing = enable != 0;
+  return SCX_OK;
+}
+
+int scx_get_timing(scx_ctx* c, uint32_t cap, const char** names, double* ms, uint32_t* count) {
+  if (!c) return SCX_E_ARG;
+  const uint32_t k = (uint32_t)c->phase_ms.size();
+  if (count) *count = k;
+  for (uint32_t i = 0; i < k && i < cap; ++i) {
+    if (names) names[i] = c->phase_names[i];
+    if (ms) ms[i] = c->phase_ms[i];
+  }
+  return SCX_OK;
+}
+
+}  // extern "C"

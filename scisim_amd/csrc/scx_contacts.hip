@@ -1,0 +1,422 @@
+// scx_contacts.hip -- active-set assembly after the broad phase: contact frames,
+// Delassus diagonal blocks and forcing terms (reference order), the deterministic
+// graph colouring of the contact graph and the colour-major record layout used by
+// the Gauss-Seidel sweeps.
+//
+// Contact frames and arms are computed in FP64 without contraction so that the
+// 3D tangent choice (a branch on |n x u|^2 < 1e-9) rounds like the CPU restatement.
+#pragma clang fp contract(off)
+
+#include <hip/hip_cooperative_groups.h>
+
+#include "scx_device.h"
+#include "scx_internal.h"
+
+namespace cg = cooperative_groups;
+
+namespace scx {
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <int D>
+__device__ __forceinline__ void normalize(double* v) {
+  double z = v[0] * v[0];
+#pragma unroll
+  for (int a = 1; a < D; ++a) z = z + v[a] * v[a];
+  if (z > 0.0) {
+    const double s = sqrt(z);
+#pragma unroll
+    for (int a = 0; a < D; ++a) v[a] = v[a] / s;
+  }
+}
+
+// FrictionUtilities::orthogonalVector (rigidbody3d/Constraints/FrictionUtilities.cpp:19-29).
+__device__ __forceinline__ void orthogonal_vector(const double* n, double* o) {
+  double e[3] = {0.0, 0.0, 0.0};
+  if (fabs(n[0]) <= fabs(n[1]) && fabs(n[0]) <= fabs(n[2]))
+    e[0] = 1.0;
+  else if (fabs(n[1]) <= fabs(n[2]))
+    e[1] = 1.0;
+  else
+    e[2] = 1.0;
+  const double d = n[0] * e[0] + n[1] * e[1] + n[2] * e[2];
+  for (int a = 0; a < 3; ++a) o[a] = e[a] - d * n[a];
+  normalize<3>(o);
+}
+
+// Tangent s opposing the tangential relative velocity u at the contact, or the
+// least-aligned axis if |n x u|^2 < 1e-9; t = normalize(n x s)
+// (SphereSphereConstraint.cpp:273-300, StaticPlaneSphereConstraint.cpp:255-283).
+__device__ __forceinline__ void frame3(const double* nn, double* s, double* t) {
+  double nxs[3];
+  cross3(nn, s, nxs);
+  if (nxs[0] * nxs[0] + nxs[1] * nxs[1] + nxs[2] * nxs[2] < 1.0e-9) {
+    orthogonal_vector(nn, s);
+  } else {
+    const double d = s[0] * nn[0] + s[1] * nn[1] + s[2] * nn[2];
+    for (int a = 0; a < 3; ++a) s[a] = s[a] - d * nn[a];
+    normalize<3>(s);
+  }
+  for (int a = 0; a < 3; ++a) s[a] *= -1.0;
+  cross3(nn, s, t);
+  normalize<3>(t);
+}
+
+struct SetupArgs {
+  uint32_t nbb, npl;
+  const uint32_t* plane_body;    // plane contact k - nbb -> body (plane-major order)
+  const uint32_t* plane_off;     // nplanes + 1 offsets into plane_body
+  uint32_t nplanes;
+  const double* q0;
+  const double* v0;
+  const double* w0;
+  const double* r;
+  const double* m;
+  const double* inertia;
+  const double* pn;
+  double CoR;
+  uint32_t* ca;
+  uint32_t* cb;
+  int32_t* cplane;
+  double* basis;
+  double* la;
+  double* lb;
+  double* wn;
+  double* wt;
+  double* forcing;
+};
+
+// Contact k of the active set, in the reference's order: body pairs first
+// (lexicographic), then plane contacts plane-major
+// (Ball2DSim::computeActiveSet ball2d/Ball2DSim.cpp:151-173;
+//  RigidBody3DSim::computeActiveSet rigidbody3d/RigidBody3DSim.cpp:250-262).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_contact_setup(SetupArgs A) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t C = A.nbb + A.npl;
+  if (k >= C) return;
+  uint32_t a, b;
+  int32_t plane = -1;
+  if (k < A.nbb) {
+    a = A.ca[k];
+    b = A.cb[k];
+  } else {
+    const uint32_t kp = k - A.nbb;
+    a = A.plane_body[kp];
+    b = kNoBody;
+    plane = 0;
+    while ((uint32_t)(plane + 1) < A.nplanes && A.plane_off[plane + 1] <= kp) ++plane;
+    A.ca[k] = a;
+    A.cb[k] = b;
+  }
+  A.cplane[k] = plane;
+  const double minv_a = 1.0 / A.m[a];
+  const double minv_b = (b != kNoBody) ? 1.0 / A.m[b] : 0.0;
+  double nn[3] = {0.0, 0.0, 0.0};
+  if (b != kNoBody) {
+    for (int d = 0; d < D; ++d) nn[d] = A.q0[(size_t)D * a + d] - A.q0[(size_t)D * b + d];
+    normalize<D>(nn);
+  } else {
+    for (int d = 0; d < D; ++d) nn[d] = A.pn[(size_t)D * plane + d];
+  }
+  double ndotv = 0.0;
+  for (int d = 0; d < D; ++d) {
+    const double dv = A.v0[(size_t)D * a + d] - ((b != kNoBody) ? A.v0[(size_t)D * b + d] : 0.0);
+    ndotv = (d == 0) ? nn[0] * dv : ndotv + nn[d] * dv;
+  }
+  A.forcing[k] = A.CoR * ndotv;
+  if (D == 2) {
+    // basis [n, t], t = (-n_y, n_x) (BallBallConstraint.cpp:244-258, StaticPlane.cpp:10-20)
+    double* B = A.basis + 4 * (size_t)k;
+    B[0] = nn[0];
+    B[1] = nn[1];
+    B[2] = -nn[1];
+    B[3] = nn[0];
+    // W_ii = (1/ma + 1/mb) [n;t][n;t]^T, diagonal entries
+    const double wn = (nn[0] * minv_a * nn[0] + nn[1] * minv_a * nn[1]) +
+                      ((b != kNoBody) ? (nn[0] * minv_b * nn[0] + nn[1] * minv_b * nn[1]) : 0.0);
+    const double wt = (nn[1] * minv_a * nn[1] + nn[0] * minv_a * nn[0]) +
+                      ((b != kNoBody) ? (nn[1] * minv_b * nn[1] + nn[0] * minv_b * nn[0]) : 0.0);
+    A.wn[k] = wn;
+    A.wt[k] = wt;
+    A.la[k] = 0.0;
+    A.lb[k] = 0.0;
+  } else {
+    double pa[3], pb[3], s[3], t[3];
+    const double* xa = A.q0 + 3 * (size_t)a;
+    const double* va = A.v0 + 3 * (size_t)a;
+    const double* wa = A.w0 + 3 * (size_t)a;
+    double ua[3];
+    if (b != kNoBody) {
+      // contact point at q0 (RigidBody3DSim.cpp:803-809)
+      const double* xb = A.q0 + 3 * (size_t)b;
+      const double f = A.r[a] / (A.r[a] + A.r[b]);
+      for (int d = 0; d < 3; ++d) {
+        const double p = xa[d] + f * (xb[d] - xa[d]);
+        pa[d] = p - xa[d];
+        pb[d] = p - xb[d];
+      }
+      double cb_[3];
+      cross3(wa, pa, ua);
+      cross3(A.w0 + 3 * (size_t)b, pb, cb_);
+      for (int d = 0; d < 3; ++d) s[d] = ((va[d] + ua[d]) - A.v0[3 * (size_t)b + d]) - cb_[d];
+    } else {
+      for (int d = 0; d < 3; ++d) {
+        pa[d] = (-A.r[a]) * nn[d];
+        pb[d] = 0.0;
+      }
+      cross3(wa, pa, ua);
+      for (int d = 0; d < 3; ++d) s[d] = (va[d] + ua[d]) - 0.0;
+    }
+    frame3(nn, s, t);
+    double* B = A.basis + 9 * (size_t)k;
+    for (int d = 0; d < 3; ++d) {
+      B[d] = nn[d];
+      B[3 + d] = s[d];
+      B[6 + d] = t[d];
+    }
+    // arms are parallel to n: r_a = -la n, r_b = +lb n
+    const double la = sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]);
+    const double lb = sqrt(pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2]);
+    const double Iinv_a = 1.0 / A.inertia[a];
+    const double Iinv_b = (b != kNoBody) ? 1.0 / A.inertia[b] : 0.0;
+    A.la[k] = la;
+    A.lb[k] = lb;
+    A.wn[k] = minv_a + minv_b;
+    A.wt[k] = (minv_a + Iinv_a * la * la) + ((b != kNoBody) ? (minv_b + Iinv_b * lb * lb) : 0.0);
+  }
+}
+
+hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double mu) {
+  (void)mu;
+  const uint32_t C = nbb + npl;
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  SetupArgs A;
+  A.nbb = nbb;
+  A.npl = npl;
+  A.plane_body = c->d.perm;  // plane contacts' bodies, written by the plane compaction
+  A.plane_off = c->d.coloff; // plane offsets, likewise
+  A.nplanes = c->nplanes;
+  A.q0 = c->d.q0;
+  A.v0 = c->d.v0;
+  A.w0 = c->d.w0;
+  A.r = c->d.r;
+  A.m = c->d.m;
+  A.inertia = c->d.inertia;
+  A.pn = c->d.pn;
+  A.CoR = CoR;
+  A.ca = c->d.ca;
+  A.cb = c->d.cb;
+  A.cplane = c->d.cplane;
+  A.basis = c->d.basis;
+  A.la = c->d.cla;
+  A.lb = c->d.clb;
+  A.wn = c->d.cwn;
+  A.wt = c->d.cwt;
+  A.forcing = c->d.cforcing;
+  if (c->dim == 2)
+    k_contact_setup<2><<<nblk, kBlock, 0, c->stream>>>(A);
+  else
+    k_contact_setup<3><<<nblk, kBlock, 0, c->stream>>>(A);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Body -> contact adjacency (CSR), then Jones-Plassmann colouring rounds.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_adj_count(uint32_t C, const uint32_t* __restrict__ ca,
+                                                      const uint32_t* __restrict__ cb, uint32_t* __restrict__ deg) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  atomicAdd(&deg[ca[k]], 1u);
+  const uint32_t b = cb[k];
+  if (b != kNoBody) atomicAdd(&deg[b], 1u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_adj_fill(uint32_t C, const uint32_t* __restrict__ ca,
+                                                     const uint32_t* __restrict__ cb, uint32_t* __restrict__ deg,
+                                                     const uint32_t* __restrict__ off, uint32_t* __restrict__ adj) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t a = ca[k];
+  adj[off[a] + atomicSub(&deg[a], 1u) - 1u] = k;
+  const uint32_t b = cb[k];
+  if (b != kNoBody) adj[off[b] + atomicSub(&deg[b], 1u) - 1u] = k;
+}
+
+// Colouring specification (shared with the CPU restatement): in round k every
+// uncoloured contact whose priority key exceeds that of every uncoloured neighbour
+// takes the smallest colour unused by its coloured neighbours. Neighbours coloured
+// during round k still count as uncoloured in round k, so decisions never depend
+// on scheduling. state = (round << 8) | colour, or -1.
+__global__ void __launch_bounds__(kBlock) k_colour(uint32_t C, const uint32_t* __restrict__ ca,
+                                                   const uint32_t* __restrict__ cb, const uint32_t* __restrict__ off,
+                                                   const uint32_t* __restrict__ adj, int32_t* state,
+                                                   StepCounters* ctr) {
+  cg::grid_group grid = cg::this_grid();
+  const uint32_t gsize = gridDim.x * blockDim.x;
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t k = gtid; k < C; k += gsize) state[k] = -1;
+  grid.sync();
+  uint32_t done = 0;
+  for (uint32_t round = 0; round <= C && done < C; ++round) {
+    uint32_t mine = 0;
+    uint32_t maxcol = 0;
+    for (uint32_t c = gtid; c < C; c += gsize) {
+      if (state[c] >= 0) continue;
+      const unsigned long long kc = prio_key(c);
+      const uint32_t bodies[2] = {ca[c], cb[c]};
+      bool is_max = true;
+      unsigned long long used = 0;
+      for (int q = 0; q < 2 && is_max; ++q) {
+        if (bodies[q] == kNoBody) continue;
+        for (uint32_t e = off[bodies[q]]; e < off[bodies[q] + 1]; ++e) {
+          const uint32_t d = adj[e];
+          if (d == c) continue;
+          const int32_t sd = __atomic_load_n(&state[d], __ATOMIC_RELAXED);
+          const bool unc = (sd < 0) || ((uint32_t)sd >> 8) == round;
+          if (unc) {
+            if (prio_key(d) > kc) {
+              is_max = false;
+              break;
+            }
+          } else {
+            used |= 1ull << (sd & 0xff);
+          }
+        }
+      }
+      if (!is_max) continue;
+      const int col = (used == ~0ull) ? 64 : __ffsll((long long)~used) - 1;
+      if (col >= kMaxColours) {
+        atomicOr(&ctr->overflow, 8u);
+        __atomic_store_n(&state[c], (int32_t)((round << 8) | 63), __ATOMIC_RELAXED);
+      } else {
+        __atomic_store_n(&state[c], (int32_t)((round << 8) | col), __ATOMIC_RELAXED);
+        maxcol = max(maxcol, (uint32_t)col + 1);
+      }
+      ++mine;
+    }
+    mine = block_reduce_sum_u32(mine);
+    if (threadIdx.x == 0 && mine) atomicAdd(&ctr->remaining, mine);
+    if (maxcol) atomicMax(&ctr->ncolours, maxcol);
+    grid.sync();
+    done = __atomic_load_n(&ctr->remaining, __ATOMIC_RELAXED);
+    if (gtid == 0 && done >= C) ctr->nrounds = round + 1;
+    // every workgroup has read the count before any adds to it again, so all of them
+    // take the same exit decision
+    grid.sync();
+  }
+}
+
+// Colour masks for the stable colour-major ordering.
+__global__ void __launch_bounds__(kBlock) k_colour_masks(uint32_t C, const int32_t* __restrict__ state,
+                                                         unsigned long long* __restrict__ mask) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  mask[k] = 1ull << (state[k] & 0xff);
+}
+
+hipError_t launch_colouring(scx_ctx* c, uint32_t C) {
+  hipStream_t s = c->stream;
+  const uint32_t n = c->n;
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  uint32_t* deg = c->d.deg;            // n counts
+  uint32_t* off = c->d.deg + n + 1;    // n + 1 offsets
+  hipError_t e = hipMemsetAsync(deg, 0, sizeof(uint32_t) * (n + 1), s);
+  if (e != hipSuccess) return e;
+  if (nblk) k_adj_count<<<nblk, kBlock, 0, s>>>(C, c->d.ca, c->d.cb, deg);
+  e = launch_exclusive_scan(deg, off, n, c->d.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  if (nblk) k_adj_fill<<<nblk, kBlock, 0, s>>>(C, c->d.ca, c->d.cb, deg, off, c->d.adj);
+  if (C == 0) return hipGetLastError();
+  const int blocks = (int)std::min<uint32_t>((uint32_t)c->colour_blocks, nblk);
+  uint32_t Cv = C;
+  const uint32_t* pa = c->d.ca;
+  const uint32_t* pb = c->d.cb;
+  const uint32_t* poff = off;
+  const uint32_t* padj = c->d.adj;
+  int32_t* pstate = c->d.cstate;
+  StepCounters* pctr = c->d.ctr;
+  void* args[] = {&Cv, &pa, &pb, &poff, &padj, &pstate, &pctr};
+  return hipLaunchCooperativeKernel((const void*)k_colour, dim3(blocks), dim3(kBlock), args, 0, s);
+}
+
+int colour_max_blocks(scx_ctx* c) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_colour, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return per_cu * c->num_cus;
+}
+
+// perm: colour order -> reference order (stable), coloff: colour offsets.
+hipError_t launch_colour_sort(scx_ctx* c, uint32_t C) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  k_colour_masks<<<nblk, kBlock, 0, c->stream>>>(C, c->d.cstate, c->d.pmask);
+  return launch_multibucket(c, c->d.pmask, C, kMaxColours, c->d.perm, c->cap_perm, c->d.coloff);
+}
+
+// Records in colour order for the sweeps (FP64, and FP32 when requested).
+__global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ ca, const uint32_t* __restrict__ cb,
+                                                   const double* __restrict__ basis, const double* __restrict__ la,
+                                                   const double* __restrict__ lb, const double* __restrict__ wn,
+                                                   const double* __restrict__ wt, const double* __restrict__ forcing,
+                                                   const double* __restrict__ rinit, double mu, ContactSoA<double> o64,
+                                                   ContactSoA<float> o32, int fp32) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t c = perm[k];
+  o64.a[k] = ca[c];
+  o64.b[k] = cb[c];
+  double n[3] = {0, 0, 0}, s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+  const int DD = dim * dim;
+  for (int d = 0; d < dim; ++d) {
+    n[d] = basis[(size_t)DD * c + d];
+    if (dim == 3) s[d] = basis[(size_t)DD * c + 3 + d];
+    r[d] = rinit ? rinit[(size_t)dim * c + d] : 0.0;
+  }
+  for (int d = 0; d < 3; ++d) {
+    o64.n[3 * (size_t)k + d] = n[d];
+    o64.s[3 * (size_t)k + d] = s[d];
+    o64.r[3 * (size_t)k + d] = r[d];
+  }
+  o64.la[k] = la[c];
+  o64.lb[k] = lb[c];
+  o64.wn[k] = wn[c];
+  o64.wt[k] = wt[c];
+  o64.mu[k] = mu;
+  o64.forcing[k] = forcing[c];
+  if (fp32) {
+    o32.a[k] = ca[c];
+    o32.b[k] = cb[c];
+    for (int d = 0; d < 3; ++d) {
+      o32.n[3 * (size_t)k + d] = (float)n[d];
+      o32.s[3 * (size_t)k + d] = (float)s[d];
+      o32.r[3 * (size_t)k + d] = (float)r[d];
+    }
+    o32.la[k] = (float)la[c];
+    o32.lb[k] = (float)lb[c];
+    o32.wn[k] = (float)wn[c];
+    o32.wt[k] = (float)wt[c];
+    o32.mu[k] = (float)mu;
+    o32.forcing[k] = (float)forcing[c];
+  }
+}
+
+hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* rinit) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  k_gather<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, c->d.perm, c->d.ca, c->d.cb, c->d.basis, c->d.cla, c->d.clb,
+                                           c->d.cwn, c->d.cwt, c->d.cforcing, rinit, mu, c->d.c64, c->d.c32,
+                                           fp32 ? 1 : 0);
+  return hipGetLastError();
+}
+
+}  // namespace scx

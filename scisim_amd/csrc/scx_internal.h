@@ -1,0 +1,178 @@
+// scx_internal.h -- device data layout, context and launch helpers shared by the
+// HIP translation units of libscx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/scx.h"
+
+namespace scx {
+
+constexpr int kBlock = 256;           // threads per workgroup (4 wave64s)
+constexpr int kScanItems = 8;         // items per thread in scans
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr uint32_t kNoBody = 0xFFFFFFFFu;
+constexpr int kMaxColours = 64;
+
+// Grid parameters, computed on the device each step (reference formula of
+// ball2d/SpatialGridDetector.cpp:76-104: min corner - 2e-6, h = max axis of the summed
+// extents / n). Device-resident so that no host round trip is needed.
+struct GridParams {
+  double min_coord[3];
+  double h;
+  uint32_t table_mask;   // hash table size - 1
+  uint32_t pad;
+};
+
+// Device-side counters of one step, read back by the host at the sync points.
+struct StepCounters {
+  unsigned long long nentries;   // grid entries
+  uint32_t overflow;             // bit 0 entries, bit 1 pairs, bit 2 contacts, bit 3 colours
+  uint32_t ncolours;
+  uint32_t nrounds;
+  uint32_t remaining;            // uncoloured contacts
+  uint32_t sweeps;               // total sweeps
+  uint32_t sweeps_fp32;
+  double error;                  // best residual of the last phase
+  double error_fp32;
+};
+
+// Per-contact record in colour order used by the sweeps. Real = float or double.
+// 2D: t = (-n_y, n_x); 3D: t = n x s; arms are r_a = -la n, r_b = +lb n (sphere arms are
+// parallel to the normal: rigidbody3d/Constraints/SphereSphereConstraint.cpp:132,253).
+template <typename Real>
+struct ContactSoA {
+  uint32_t* a;
+  uint32_t* b;
+  Real* n;      // 3 per contact (2D: 2 used, stride 3 kept for one layout)
+  Real* s;      // 3 per contact (3D)
+  Real* la;
+  Real* lb;
+  Real* wn;     // W_nn (Delassus diagonal, normal)
+  Real* wt;     // W_tt = W_ss (tangential)
+  Real* mu;
+  Real* forcing;
+  Real* r;      // 3 per contact
+};
+
+struct DeviceBuffers {
+  // bodies
+  double* q = nullptr;       // dim*N
+  double* v = nullptr;       // dim*N
+  double* w = nullptr;       // 3*N (3D)
+  double* r = nullptr;       // N
+  double* m = nullptr;       // N
+  double* inertia = nullptr; // N (3D)
+  double* q0 = nullptr;      // dim*N start-of-step copy
+  double* v0 = nullptr;
+  double* w0 = nullptr;
+  double* q1p = nullptr;     // predictor positions
+  double* v2 = nullptr;      // post-impact velocity (dim*N)
+  double* w2 = nullptr;      // (3D)
+  // working velocities for the sweeps: per body [v (3) | minv] and [omega (3) | Iinv]
+  double* bv64 = nullptr;    // 4*N
+  double* bw64 = nullptr;    // 4*N
+  float* bv32 = nullptr;
+  float* bw32 = nullptr;
+  // planes
+  double* px = nullptr;      // dim*P
+  double* pn = nullptr;      // dim*P (normalised)
+  // broad phase
+  double* aabb_min = nullptr;  // dim*N
+  double* aabb_max = nullptr;
+  double* red = nullptr;       // reduction partials
+  GridParams* grid = nullptr;
+  uint32_t* bucket = nullptr;  // table_size + 1 counts / starts
+  unsigned long long* ekey = nullptr;  // entries: packed cell key | first-cell flags
+  uint32_t* eid = nullptr;
+  uint32_t* pcount = nullptr;  // N + 1 per-owner counts / offsets
+  uint32_t* scan_tmp = nullptr; // scratch of the device-wide scans
+  uint32_t* pa = nullptr;      // pair lists of scx_potential_overlaps (capacity cap_pairs)
+  uint32_t* pb = nullptr;
+  // active set in reference order
+  uint32_t* ca = nullptr;      // body a
+  uint32_t* cb = nullptr;      // body b or kNoBody
+  int32_t* cplane = nullptr;   // plane index or -1
+  unsigned long long* pmask = nullptr;  // per-body active plane mask / per-contact colour mask
+  uint32_t* mbcount = nullptr; // multi-bucket counts
+  // contact setup in reference order (FP64)
+  double* basis = nullptr;     // dim*dim per contact
+  double* cla = nullptr;
+  double* clb = nullptr;
+  double* cwn = nullptr;
+  double* cwt = nullptr;
+  double* cforcing = nullptr;
+  double* calpha = nullptr;    // outputs in reference order
+  double* cbeta = nullptr;
+  double* crinit = nullptr;    // warm start impulses (local) in reference order
+  // colouring
+  uint32_t* deg = nullptr;     // N + 1 adjacency offsets
+  uint32_t* adj = nullptr;     // 2C
+  int32_t* cstate = nullptr;   // (round << 8) | colour, -1 uncoloured
+  uint32_t* perm = nullptr;    // colour order -> reference order
+  uint32_t* coloff = nullptr;  // colour offsets (kMaxColours + 1)
+  // sweep records
+  ContactSoA<double> c64{};
+  ContactSoA<float> c32{};
+  double* rbest = nullptr;     // 3 per contact
+  double* partial = nullptr;   // per-block residual partials
+  // warm-start cache (previous step's active set, sorted keys + impulses)
+  unsigned long long* prev_key = nullptr;
+  double* prev_r = nullptr;    // dim per contact, local
+  uint32_t prev_count = 0;
+  unsigned long long* cur_key = nullptr;
+  StepCounters* ctr = nullptr;
+  StepCounters* ctr_host = nullptr;  // pinned
+  uint32_t* host_u32 = nullptr;      // pinned scratch
+};
+
+}  // namespace scx
+
+struct scx_ctx {
+  int dim = 0;
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint32_t cap_bodies = 0, cap_planes = 0, cap_contacts = 0, cap_entries = 0, cap_pairs = 0, table_size = 0;
+  uint32_t cap_perm = 0;  // perm buffer length (plane compaction and colour order)
+  uint32_t n = 0, nplanes = 0;
+  bool has_bodies = false;
+  uint32_t last_ncontacts = 0, last_npairs = 0, last_nplane = 0, last_ncolours = 0;
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  std::vector<const char*> event_names;
+  std::vector<double> phase_ms;
+  std::vector<const char*> phase_names;
+  int gs_blocks = 0;
+  int colour_blocks = 0;
+  scx::DeviceBuffers d;
+};
+
+// Launch wrappers implemented in the .hip translation units.
+namespace scx {
+hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s);
+size_t scan_tmp_words(uint32_t n);
+
+hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, const double* m, double dt,
+                               const double* g, int integrator, double* q1p);
+hipError_t launch_flow(scx_ctx* c, uint32_t n, int integrator, const double* m, const double* g, double dt,
+                       const double* q0, const double* v0, double* q1, double* v1);
+hipError_t launch_grid_build(scx_ctx* c, uint32_t n);
+hipError_t launch_pairs(scx_ctx* c, uint32_t n, int mode, int filter, uint32_t* out_a, uint32_t* out_b, uint32_t cap);
+hipError_t launch_plane_masks(scx_ctx* c, uint32_t n);
+hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint32_t n, uint32_t nbuckets,
+                              uint32_t* out_idx, uint32_t out_cap, uint32_t* bucket_offsets);
+hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double mu);
+hipError_t launch_colouring(scx_ctx* c, uint32_t C);
+hipError_t launch_colour_sort(scx_ctx* c, uint32_t C);
+hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* rinit);
+hipError_t launch_solve(scx_ctx* c, uint32_t C, const scx_step_params* p);
+hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p);
+int solver_max_blocks(scx_ctx* c);
+int colour_max_blocks(scx_ctx* c);
+}  // namespace scx

@@ -1,0 +1,539 @@
+// scx_solver.hip -- graph-coloured projected Gauss-Seidel Coulomb-friction solve
+// (the So-bogus GaussSeidel + SOCLaw<D,double,true> path of
+// scisim/ConstrainedMaps/bogus/Ball2DSobogusInterface.cpp:102-148 and
+// RigidBody3DSobogusInterface.cpp:99-145), Fischer-Burmeister residual, and the
+// end-of-step update (GeometricImpactFrictionMap.cpp:441).
+//
+// One persistent cooperative launch runs the whole solve: per sweep, one pass per
+// colour (contacts of a colour share no body, so their updates are independent and
+// the result does not depend on scheduling), a grid barrier between colours, and the
+// residual (max over contacts of ||FB||^2, in FP64) every eval_every sweeps; all
+// workgroups read the same reduced residual after a barrier and leave the loop
+// together. The sweep works in primal-velocity form: the body velocities
+// v = v0 + M^-1 H^T r are kept current, so a contact's pre-impulse velocity is
+// H v + w and no Delassus off-diagonal block is ever formed.
+#pragma clang fp contract(fast)
+
+#include <hip/hip_cooperative_groups.h>
+
+#include "scx_device.h"
+#include "scx_flow.h"
+#include "scx_internal.h"
+
+namespace cg = cooperative_groups;
+
+namespace scx {
+
+template <typename Real>
+struct Vec4 {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+
+template <int D, typename Real>
+struct Rec {
+  uint32_t a, b;
+  Real n[3], s[3], t[3];
+  Real la, lb, wn, wt, mu, f;
+};
+
+template <int D, typename Real>
+__device__ __forceinline__ void load_rec(const ContactSoA<Real>& c, uint32_t k, Rec<D, Real>& R) {
+  R.a = c.a[k];
+  R.b = c.b[k];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) R.n[d] = c.n[3 * (size_t)k + d];
+  if (D == 3) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) R.s[d] = c.s[3 * (size_t)k + d];
+    R.t[0] = R.n[1] * R.s[2] - R.n[2] * R.s[1];
+    R.t[1] = R.n[2] * R.s[0] - R.n[0] * R.s[2];
+    R.t[2] = R.n[0] * R.s[1] - R.n[1] * R.s[0];
+    R.la = c.la[k];
+    R.lb = c.lb[k];
+  } else {
+    R.t[0] = -R.n[1];
+    R.t[1] = R.n[0];
+    R.t[2] = 0;
+    R.s[0] = R.s[1] = R.s[2] = 0;
+    R.la = R.lb = 0;
+  }
+  R.wn = c.wn[k];
+  R.wt = c.wt[k];
+  R.mu = c.mu[k];
+  R.f = c.forcing[k];
+}
+
+// Pre-impulse relative velocity in the contact frame, u = H_a v_a - H_b v_b + w,
+// computed in precision T from bodies stored in precision Real.
+template <int D, typename T, typename Real>
+__device__ __forceinline__ void rel_vel(const Rec<D, Real>& R, const typename Vec4<Real>::type& va,
+                                        const typename Vec4<Real>::type& wa, const typename Vec4<Real>::type& vb,
+                                        const typename Vec4<Real>::type& wb, T u[3]) {
+  const T dv[3] = {(T)va.x - (T)vb.x, (T)va.y - (T)vb.y, (T)va.z - (T)vb.z};
+  const T n[3] = {(T)R.n[0], (T)R.n[1], (T)R.n[2]};
+  const T t[3] = {(T)R.t[0], (T)R.t[1], (T)R.t[2]};
+  u[0] = n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2] + (T)R.f;
+  if (D == 3) {
+    const T s[3] = {(T)R.s[0], (T)R.s[1], (T)R.s[2]};
+    const T la = (T)R.la, lb = (T)R.lb;
+    const T ca[3] = {(T)wa.y * n[2] - (T)wa.z * n[1], (T)wa.z * n[0] - (T)wa.x * n[2],
+                     (T)wa.x * n[1] - (T)wa.y * n[0]};
+    const T cb[3] = {(T)wb.y * n[2] - (T)wb.z * n[1], (T)wb.z * n[0] - (T)wb.x * n[2],
+                     (T)wb.x * n[1] - (T)wb.y * n[0]};
+    T uv[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uv[d] = dv[d] - la * ca[d] - lb * cb[d];
+    u[1] = s[0] * uv[0] + s[1] * uv[1] + s[2] * uv[2];
+    u[2] = t[0] * uv[0] + t[1] * uv[1] + t[2] * uv[2];
+  } else {
+    u[1] = t[0] * dv[0] + t[1] * dv[1];
+    u[2] = 0;
+  }
+}
+
+// Exact Coulomb solution for the block diag(w_n, w_t, w_t): separate, stick, or slide
+// on the cone boundary against the tangential pre-velocity.
+template <int D, typename Real>
+__device__ __forceinline__ void local_solve(const Rec<D, Real>& R, const Real u[3], const Real r[3], Real rn[3]) {
+  const Real bn = u[0] - R.wn * r[0];
+  const Real n0 = -bn / R.wn;
+  if (!(n0 > (Real)0)) {
+    rn[0] = rn[1] = rn[2] = 0;
+    return;
+  }
+  rn[0] = n0;
+  Real nrm2 = 0;
+#pragma unroll
+  for (int a = 1; a < D; ++a) {
+    rn[a] = -(u[a] - R.wt * r[a]) / R.wt;
+    nrm2 += rn[a] * rn[a];
+  }
+  if (D == 2) rn[2] = 0;
+  const Real lim = R.mu * n0;
+  const Real nrm = sqrt(nrm2);
+  if (nrm > lim) {
+    const Real sc = (nrm > (Real)0) ? lim / nrm : (Real)0;
+#pragma unroll
+    for (int a = 1; a < D; ++a) rn[a] *= sc;
+  }
+}
+
+// FB residual of one contact (De Saxce change of variable), ||phi||^2, in FP64.
+template <int D>
+__device__ __forceinline__ double fb_err(double mu, const double* r, const double* u) {
+  double uT2 = 0.0;
+#pragma unroll
+  for (int a = 1; a < D; ++a) uT2 += u[a] * u[a];
+  const double un_hat = u[0] + mu * sqrt(uT2);
+  double xh[3], yh[3];
+  xh[0] = mu * r[0];
+  yh[0] = un_hat;
+#pragma unroll
+  for (int a = 1; a < D; ++a) {
+    xh[a] = r[a];
+    yh[a] = mu * u[a];
+  }
+  double z0 = 0.0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) z0 += xh[a] * xh[a] + yh[a] * yh[a];
+  double z1[2] = {0.0, 0.0}, nz1 = 0.0;
+#pragma unroll
+  for (int a = 1; a < D; ++a) {
+    z1[a - 1] = 2.0 * (xh[0] * xh[a] + yh[0] * yh[a]);
+    nz1 += z1[a - 1] * z1[a - 1];
+  }
+  nz1 = sqrt(nz1);
+  const double s1 = sqrt(z0 + nz1);
+  const double s2 = sqrt(fmax(0.0, z0 - nz1));
+  double e = xh[0] + yh[0] - 0.5 * (s1 + s2);
+  e = e * e;
+#pragma unroll
+  for (int a = 1; a < D; ++a) {
+    const double dir = nz1 > 0.0 ? z1[a - 1] / nz1 : 0.0;
+    const double p = xh[a] + yh[a] - 0.5 * (s1 - s2) * dir;
+    e += p * p;
+  }
+  return e;
+}
+
+template <typename Real>
+struct GSArgs {
+  ContactSoA<Real> c;
+  Real* bv;                 // 4 per body: v (3), minv
+  Real* bw;                 // 4 per body: omega (3), Iinv
+  const double* v0;         // dim per body
+  const double* w0;         // 3 per body (3D)
+  uint32_t nbodies;
+  uint32_t C;
+  const uint32_t* coloff;   // colour offsets
+  Real* rbest;              // 3 per contact
+  double* partial;          // per-workgroup residual partials
+  StepCounters* ctr;
+  uint32_t max_iters;
+  uint32_t eval_every;
+  double tol;
+  int phase;                // 0: only phase, 1: FP32 phase, 2: FP64 polish after FP32
+  int apply_init;           // reset velocities to v0 and apply the records' impulses
+};
+
+template <int D, typename Real>
+__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init) {
+  using V = typename Vec4<Real>::type;
+  Rec<D, Real> R;
+  load_rec<D, Real>(A.c, k, R);
+  const bool hasb = R.b != kNoBody;
+  V* bv = reinterpret_cast<V*>(A.bv);
+  V* bw = reinterpret_cast<V*>(A.bw);
+  V va = bv[R.a], wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
+  if (D == 3) wa = bw[R.a];
+  if (hasb) {
+    vb = bv[R.b];
+    if (D == 3) wb = bw[R.b];
+  }
+  Real* rp = A.c.r + 3 * (size_t)k;
+  const Real r[3] = {rp[0], rp[1], rp[2]};
+  Real dr[3];
+  if (init) {
+    dr[0] = r[0];
+    dr[1] = r[1];
+    dr[2] = r[2];
+  } else {
+    Real u[3], rn[3];
+    rel_vel<D, Real, Real>(R, va, wa, vb, wb, u);
+    local_solve<D, Real>(R, u, r, rn);
+    dr[0] = rn[0] - r[0];
+    dr[1] = rn[1] - r[1];
+    dr[2] = rn[2] - r[2];
+    rp[0] = rn[0];
+    rp[1] = rn[1];
+    if (D == 3) rp[2] = rn[2];
+  }
+  Real P[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) P[d] = R.n[d] * dr[0] + R.t[d] * dr[D - 1] + ((D == 3) ? R.s[d] * dr[1] : (Real)0);
+  va.x += va.w * P[0];
+  va.y += va.w * P[1];
+  va.z += va.w * P[2];
+  bv[R.a] = va;
+  Real tau[3];
+  if (D == 3) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) tau[d] = dr[1] * R.t[d] - dr[2] * R.s[d];
+    const Real ka = -R.la * wa.w;
+    wa.x += ka * tau[0];
+    wa.y += ka * tau[1];
+    wa.z += ka * tau[2];
+    bw[R.a] = wa;
+  }
+  if (hasb) {
+    vb.x -= vb.w * P[0];
+    vb.y -= vb.w * P[1];
+    vb.z -= vb.w * P[2];
+    bv[R.b] = vb;
+    if (D == 3) {
+      const Real kb = R.lb * wb.w;
+      wb.x -= kb * tau[0];
+      wb.y -= kb * tau[1];
+      wb.z -= kb * tau[2];
+      bw[R.b] = wb;
+    }
+  }
+}
+
+template <int D, typename Real>
+__device__ __forceinline__ double contact_err(const GSArgs<Real>& A, uint32_t k) {
+  using V = typename Vec4<Real>::type;
+  Rec<D, Real> R;
+  load_rec<D, Real>(A.c, k, R);
+  const V* bv = reinterpret_cast<const V*>(A.bv);
+  const V* bw = reinterpret_cast<const V*>(A.bw);
+  const V z = {0, 0, 0, 0};
+  const V va = bv[R.a];
+  const V wa = (D == 3) ? bw[R.a] : z;
+  const V vb = (R.b != kNoBody) ? bv[R.b] : z;
+  const V wb = (D == 3 && R.b != kNoBody) ? bw[R.b] : z;
+  double u[3];
+  rel_vel<D, double, Real>(R, va, wa, vb, wb, u);
+  const Real* rp = A.c.r + 3 * (size_t)k;
+  const double r[3] = {(double)rp[0], (double)rp[1], (double)rp[2]};
+  return fb_err<D>((double)R.mu, r, u);
+}
+
+// Grid-wide max of the residual; identical in every thread after the barrier.
+template <int D, typename Real>
+__device__ double eval_residual(const GSArgs<Real>& A, cg::grid_group& grid, uint32_t gtid, uint32_t gsize) {
+  __shared__ double s_red[kBlock / 64];
+  double m = 0.0;
+  for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real>(A, k));
+  m = wave_reduce_max_f64(m);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_red[0];
+    for (int w = 1; w < kBlock / 64; ++w) b = fmax(b, s_red[w]);
+    A.partial[blockIdx.x] = b;
+  }
+  grid.sync();
+  double e = 0.0;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) e = fmax(e, A.partial[b]);
+  e = wave_reduce_max_f64(e);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  double out = s_red[0];
+  for (int w = 1; w < kBlock / 64; ++w) out = fmax(out, s_red[w]);
+  __syncthreads();
+  return out;
+}
+
+// v = v0 + M^-1 H^T r for the current impulses, applied colour by colour. Only bodies
+// touched by a contact can differ from v0 (every body starts at v0, k_body_init), so
+// a reset touches those bodies only; two contacts sharing a body write the same v0.
+template <int D, typename Real>
+__device__ void apply_impulses(const GSArgs<Real>& A, cg::grid_group& grid, const uint32_t* off, uint32_t ncol,
+                               uint32_t gtid, uint32_t gsize, bool reset) {
+  if (reset) {
+    for (uint32_t k = gtid; k < A.C; k += gsize) {
+      const uint32_t bodies[2] = {A.c.a[k], A.c.b[k]};
+      for (int q = 0; q < 2; ++q) {
+        const uint32_t i = bodies[q];
+        if (i == kNoBody) continue;
+#pragma unroll
+        for (int d = 0; d < D; ++d) A.bv[4 * (size_t)i + d] = (Real)A.v0[(size_t)D * i + d];
+        if (D == 3)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) A.bw[4 * (size_t)i + d] = (Real)A.w0[3 * (size_t)i + d];
+      }
+    }
+    grid.sync();
+  }
+  for (uint32_t col = 0; col < ncol; ++col) {
+    for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, true);
+    grid.sync();
+  }
+}
+
+template <int D, typename Real>
+__global__ void __launch_bounds__(kBlock) k_gs(GSArgs<Real> A) {
+  cg::grid_group grid = cg::this_grid();
+  const uint32_t gsize = gridDim.x * kBlock;
+  const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ uint32_t off[kMaxColours + 1];
+  const uint32_t ncol = A.ctr->ncolours;
+  for (uint32_t c = threadIdx.x; c <= ncol; c += kBlock) off[c] = A.coloff[c];
+  __syncthreads();
+  uint32_t max_iters = A.max_iters;
+  // FP32 phase: at most half the sweep budget, the FP64 polish gets the rest
+  if (A.phase == 1) max_iters = max(1u, max_iters / 2);
+  if (A.phase == 2) max_iters = (A.ctr->sweeps_fp32 < max_iters) ? max_iters - A.ctr->sweeps_fp32 : 0u;
+  const bool track_best = A.phase != 1;
+
+  if (A.apply_init) apply_impulses<D, Real>(A, grid, off, ncol, gtid, gsize, false);
+
+  double err = eval_residual<D, Real>(A, grid, gtid, gsize);
+  double best = err;
+  if (track_best) {
+    for (uint32_t k = gtid; k < A.C; k += gsize)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
+    grid.sync();
+  }
+  // FP32 phase stagnation: leave when 64 sweeps did not halve the best residual
+  double ref = best;
+  uint32_t ref_it = 0;
+  uint32_t it = 0;
+  bool stagnated = false;
+  if (!(err < A.tol)) {
+    for (it = 1; it <= max_iters; ++it) {
+      for (uint32_t col = 0; col < ncol; ++col) {
+        for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false);
+        grid.sync();
+      }
+      if (A.eval_every > 0 && it % A.eval_every == 0) {
+        err = eval_residual<D, Real>(A, grid, gtid, gsize);
+        if (err < best) {
+          best = err;
+          if (track_best) {
+            for (uint32_t k = gtid; k < A.C; k += gsize)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
+            grid.sync();
+          }
+        }
+        if (err < A.tol) break;
+        if (A.phase == 1 && it - ref_it >= 64) {
+          if (best > 0.5 * ref) {
+            stagnated = true;
+            break;
+          }
+          ref = best;
+          ref_it = it;
+        }
+      }
+    }
+    if (it > max_iters && track_best) {
+      // not converged: return the best iterate and its velocities (So-bogus keeps x_best)
+      for (uint32_t k = gtid; k < A.C; k += gsize)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) A.c.r[3 * (size_t)k + d] = A.rbest[3 * (size_t)k + d];
+      grid.sync();
+      apply_impulses<D, Real>(A, grid, off, ncol, gtid, gsize, true);
+    }
+    if (it > max_iters) it = max_iters;
+  }
+  (void)stagnated;
+  if (gtid == 0) {
+    if (A.phase == 1) {
+      A.ctr->sweeps_fp32 = it;
+      A.ctr->error_fp32 = best;
+    }
+    A.ctr->sweeps += it;
+    A.ctr->error = best;
+  }
+}
+
+// Per-body packed [v | minv], [omega | Iinv] in both precisions.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_body_init(uint32_t n, const double* __restrict__ v0,
+                                                      const double* __restrict__ w0, const double* __restrict__ m,
+                                                      const double* __restrict__ inertia, double* __restrict__ bv64,
+                                                      double* __restrict__ bw64, float* __restrict__ bv32,
+                                                      float* __restrict__ bw32, int fp32) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) v[d] = v0[(size_t)D * i + d];
+  if (D == 3)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) w[d] = w0[3 * (size_t)i + d];
+  const double minv = 1.0 / m[i];
+  const double iinv = (D == 3) ? 1.0 / inertia[i] : 0.0;
+  reinterpret_cast<double4*>(bv64)[i] = make_double4(v[0], v[1], v[2], minv);
+  if (D == 3) reinterpret_cast<double4*>(bw64)[i] = make_double4(w[0], w[1], w[2], iinv);
+  if (fp32) {
+    reinterpret_cast<float4*>(bv32)[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)minv);
+    if (D == 3) reinterpret_cast<float4*>(bw32)[i] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)iinv);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_promote_r(uint32_t C, const float* __restrict__ r32,
+                                                      double* __restrict__ r64) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= 3 * C) return;
+  r64[k] = (double)r32[k];
+}
+
+// End of step: v2 from the sweep velocities, impulses back to the reference order,
+// and the final flow from q0 with v2 (GeometricImpactFrictionMap.cpp:441).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_finish_bodies(uint32_t n, const double* __restrict__ bv64,
+                                                          const double* __restrict__ bw64,
+                                                          const double* __restrict__ q0, const double* __restrict__ m,
+                                                          double dt, double gx, double gy, double gz, int integrator,
+                                                          double* __restrict__ q, double* __restrict__ v,
+                                                          double* __restrict__ w, double* __restrict__ v2,
+                                                          double* __restrict__ w2) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double g[3] = {gx, gy, gz};
+  const double4 bvi = reinterpret_cast<const double4*>(bv64)[i];
+  const double vv[3] = {bvi.x, bvi.y, bvi.z};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const size_t k = (size_t)D * i + d;
+    v2[k] = vv[d];
+    flow_component<D>(integrator, m[i], g[d], dt, q0[k], vv[d], &q[k], &v[k]);
+  }
+  if (D == 3) {
+    const double4 bwi = reinterpret_cast<const double4*>(bw64)[i];
+    w2[3 * (size_t)i] = w[3 * (size_t)i] = bwi.x;
+    w2[3 * (size_t)i + 1] = w[3 * (size_t)i + 1] = bwi.y;
+    w2[3 * (size_t)i + 2] = w[3 * (size_t)i + 2] = bwi.z;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, const uint32_t* __restrict__ perm,
+                                                         const double* __restrict__ r64, double* __restrict__ alpha,
+                                                         double* __restrict__ beta, double mu) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t c = perm[k];
+  alpha[c] = r64[3 * (size_t)k];
+  for (int d = 1; d < dim; ++d) beta[(size_t)(dim - 1) * c + d - 1] = (mu == 0.0) ? 0.0 : r64[3 * (size_t)k + d];
+}
+
+template <int D, typename Real>
+static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks) {
+  const uint32_t want = (std::max<uint32_t>(A.C, 1u) + kBlock - 1) / kBlock;
+  const int blocks = (int)std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)max_blocks, want));
+  void* args[] = {&A};
+  return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+}
+
+template <int D, typename Real>
+int gs_max_blocks(scx_ctx* c) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gs<D, Real>, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return per_cu * c->num_cus;
+}
+
+template <int D>
+static hipError_t solve_dim(scx_ctx* c, uint32_t C, const scx_step_params* p) {
+  hipStream_t s = c->stream;
+  const bool fp32 = p->precision == SCX_PRECISION_FP32_POLISH;
+  const bool warm = p->warm_start != SCX_WARM_NONE;
+  const uint32_t nb = (c->n + kBlock - 1) / kBlock;
+  if (nb)
+    k_body_init<D><<<nb, kBlock, 0, s>>>(c->n, c->d.v0, c->d.w0, c->d.m, c->d.inertia, c->d.bv64, c->d.bw64,
+                                          c->d.bv32, c->d.bw32, fp32 ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || C == 0) return e;
+  if (fp32) {
+    GSArgs<float> A{c->d.c32, c->d.bv32, c->d.bw32, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
+                    reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, p->max_iters,
+                    p->eval_every, p->tol, 1, warm ? 1 : 0};
+    e = coop_gs<D, float>(c, A, c->gs_blocks);
+    if (e != hipSuccess) return e;
+    const uint32_t nr = (3 * C + kBlock - 1) / kBlock;
+    k_promote_r<<<nr, kBlock, 0, s>>>(C, c->d.c32.r, c->d.c64.r);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  GSArgs<double> B{c->d.c64, c->d.bv64, c->d.bw64, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
+                   c->d.partial, c->d.ctr, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0, (fp32 || warm) ? 1 : 0};
+  return coop_gs<D, double>(c, B, c->gs_blocks);
+}
+
+hipError_t launch_solve(scx_ctx* c, uint32_t C, const scx_step_params* p) {
+  return c->dim == 2 ? solve_dim<2>(c, C, p) : solve_dim<3>(c, C, p);
+}
+
+hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {
+  hipStream_t s = c->stream;
+  const uint32_t nb = (c->n + kBlock - 1) / kBlock;
+  if (nb) {
+    if (c->dim == 2)
+      k_finish_bodies<2><<<nb, kBlock, 0, s>>>(c->n, c->d.bv64, c->d.bw64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
+                                               p->g[2], p->integrator, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
+    else
+      k_finish_bodies<3><<<nb, kBlock, 0, s>>>(c->n, c->d.bv64, c->d.bw64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
+                                               p->g[2], p->integrator, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
+  }
+  const uint32_t nc = (C + kBlock - 1) / kBlock;
+  if (nc) k_impulses_out<<<nc, kBlock, 0, s>>>(C, c->dim, c->d.perm, c->d.c64.r, c->d.calpha, c->d.cbeta, p->mu);
+  return hipGetLastError();
+}
+
+int solver_max_blocks(scx_ctx* c) {
+  return c->dim == 2 ? std::min(gs_max_blocks<2, double>(c), gs_max_blocks<2, float>(c))
+                     : std::min(gs_max_blocks<3, double>(c), gs_max_blocks<3, float>(c));
+}
+
+}  // namespace scx

@@ -1,0 +1,65 @@
+"""CPU checks of the C ABI library: it loads, exports every symbol include/scx.h
+declares, and rejects invalid configurations before touching a device."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+import scisim_amd as scx
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "scx.h")
+
+
+def declared_symbols():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int|void)\s+(scx_\w+)\s*\(", src, re.M)))
+
+
+def test_header_and_python_binding_agree():
+    assert declared_symbols() == sorted(scx.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = scx.lib()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert lib.scx_version().startswith(b"scx")
+
+
+def test_invalid_config_is_rejected_on_the_host():
+    lib = scx.lib()
+    h = ctypes.c_void_p()
+    for cfg in (scx.ScxConfig(dim=4, device=0, max_bodies=10, max_planes=4),
+                scx.ScxConfig(dim=2, device=0, max_bodies=0, max_planes=4),
+                scx.ScxConfig(dim=3, device=0, max_bodies=10, max_planes=65)):
+        assert lib.scx_create(ctypes.byref(cfg), ctypes.byref(h)) == scx.SCX_E_ARG
+        assert not h
+    assert lib.scx_create(None, ctypes.byref(h)) == scx.SCX_E_ARG
+
+
+def test_null_context_is_rejected():
+    lib = scx.lib()
+    assert lib.scx_step(None, None, None) == scx.SCX_E_ARG
+    assert lib.scx_set_timing(None, 1) == scx.SCX_E_ARG
+    assert lib.scx_last_error(None) == b"null context"
+
+
+def test_scene_generators_are_deterministic_and_non_overlapping():
+    from scisim_amd import scenes
+    a = scenes.ball2d_rsa(n=3000, seed=9)
+    b = scenes.ball2d_rsa(n=3000, seed=9)
+    assert np.array_equal(a["q"], b["q"]) and np.array_equal(a["r"], b["r"])
+    q = a["q"].reshape(-1, 2)
+    d = np.sqrt(((q[:, None, :] - q[None, :, :]) ** 2).sum(-1))
+    rs = a["r"][:, None] + a["r"][None, :]
+    np.fill_diagonal(d, np.inf)
+    assert np.all(d >= rs)
+    assert abs(np.pi * (a["r"] ** 2).sum() / a["side"] ** 2 - 0.55) < 1e-9
+    s = scenes.spheres_rsa(n=2000, seed=3)
+    q = s["q"].reshape(-1, 3)
+    d = np.sqrt(((q[:, None, :] - q[None, :, :]) ** 2).sum(-1))
+    np.fill_diagonal(d, np.inf)
+    assert np.all(d >= s["r"][:, None] + s["r"][None, :])
+    assert np.all(q[:, 1] - s["r"] >= 0.0)
