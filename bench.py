@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--settle", type=int, default=200, help="untimed settling steps (cfg2)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--max-iters", type=int, default=None, help="override the sweep limit (profiling)")
+    ap.add_argument("--eval-every", type=int, default=None, help="override the residual period (profiling)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
@@ -123,6 +125,10 @@ def main():
         ctx.run(sp, args.settle)
         log("settled %d steps in %.1fs" % (args.settle, time.perf_counter() - t0))
         it += args.settle
+    if args.max_iters is not None:
+        prm["max_iters"] = args.max_iters
+    if args.eval_every is not None:
+        prm["eval_every"] = args.eval_every
     params = scx.step_params(precision=precision, iteration=it, **prm)
     for _ in range(args.warmup):
         ctx.step(params)

@@ -7,9 +7,11 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
+#include "scx_device.h"
 #include "scx_internal.h"
 
 using namespace scx;
@@ -83,6 +85,16 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
   HIPC(dalloc(&d.pcount, (size_t)N + 1));
   HIPC(dalloc(&d.pmask, std::max<size_t>(N, c->cap_contacts)));
   HIPC(dalloc(&d.deg, 2 * ((size_t)N + 1)));
+  HIPC(dalloc(&d.sid, N));
+  HIPC(dalloc(&d.smin, (size_t)D * N));
+  HIPC(dalloc(&d.smax, (size_t)D * N));
+  HIPC(dalloc(&d.sq0, (size_t)D * N));
+  HIPC(dalloc(&d.sq1, (size_t)D * N));
+  HIPC(dalloc(&d.sr, N));
+  // dense grid budget: cells of the reference's size (mean box extent) number about N
+  // for a packed scene; the grid is coarsened on the device beyond this budget
+  c->cap_cells = 2 * N + 1024;
+  HIPC(dalloc(&d.cells, 4 * ((size_t)c->cap_cells + 1)));
   c->cap_bodies = N;
   return SCX_OK;
 }
@@ -90,10 +102,7 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
 int alloc_entries(scx_ctx* c, uint32_t cap_entries) {
   DeviceBuffers& d = c->d;
   c->cap_entries = cap_entries;
-  c->table_size = next_pow2(std::max<uint64_t>(1024, cap_entries));
-  HIPC(dalloc(&d.ekey, cap_entries));
-  HIPC(dalloc(&d.eid, cap_entries));
-  HIPC(dalloc(&d.bucket, 2 * (size_t)c->table_size + 1));
+  HIPC(dalloc(&d.ent, cap_entries));
   return SCX_OK;
 }
 
@@ -158,7 +167,7 @@ int alloc_contacts(scx_ctx* c, uint32_t C) {
 }
 
 int alloc_scan_tmp(scx_ctx* c) {
-  const size_t big = std::max<size_t>({(size_t)c->table_size, (size_t)c->cap_bodies + 1, (size_t)c->cap_pairs,
+  const size_t big = std::max<size_t>({(size_t)c->cap_cells + 1, (size_t)c->cap_bodies + 1, (size_t)c->cap_pairs,
                                        (size_t)kMaxColours * ((std::max<size_t>(c->cap_contacts, c->cap_bodies) +
                                                                kBlock - 1) / kBlock) + 2});
   HIPC(dalloc(&c->d.scan_tmp, scan_tmp_words((uint32_t)std::min<size_t>(big, 0xFFFFFFFFu)) + 4));
@@ -175,18 +184,21 @@ void free_all(scx_ctx* c) {
   float* flt[] = {d.bv32, d.bw32};
   for (float* p : flt)
     if (p) hipFree(p);
-  uint32_t* u32[] = {d.bucket, d.eid, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
+  uint32_t* u32[] = {d.cells, d.ent, d.sid, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
     if (p) hipFree(p);
   if (d.cplane) hipFree(d.cplane);
   if (d.cstate) hipFree(d.cstate);
-  if (d.ekey) hipFree(d.ekey);
+  double* sorted[] = {d.smin, d.smax, d.sq0, d.sq1, d.sr};
+  for (double* p : sorted)
+    if (p) hipFree(p);
   if (d.pmask) hipFree(d.pmask);
   if (d.prev_key) hipFree(d.prev_key);
   if (d.cur_key) hipFree(d.cur_key);
   if (d.grid) hipFree(d.grid);
   if (d.ctr) hipFree(d.ctr);
+  if (d.barrier) hipFree(d.barrier);
   if (d.ctr_host) hipHostFree(d.ctr_host);
   if (d.host_u32) hipHostFree(d.host_u32);
   free_soa(d.c64);
@@ -241,7 +253,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   HIPC(launch_predict_aabb(c, d.q0, d.v0, d.m, p->dt, p->g, p->integrator, d.q1p));
   timing_mark(c, "predict");
   // broad phase binning
-  HIPC(launch_grid_build(c, n));
+  HIPC(launch_grid_build(c, n, false, true));
   timing_mark(c, "grid");
   // body-body active set (Ball2DSim.cpp:553-608 / RigidBody3DSim.cpp:1072-1260)
   const int filter = (D == 2) ? 1 : 2;
@@ -284,16 +296,21 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     return SCX_E_CAPACITY;
   }
   const uint32_t C = (uint32_t)C64;
+  uint32_t max_colour = 0;
   if (C > 0) {
     HIPC(launch_contact_setup(c, nbb, npl, p->CoR, p->mu));
     timing_mark(c, "setup");
     HIPC(launch_colouring(c, C));
     HIPC(launch_colour_sort(c, C));
+    // second sync point: colour sizes size the persistent sweep launch
+    HIPC(hipMemcpyAsync(d.host_u32 + 2, d.coloff, sizeof(uint32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    for (int b = 0; b < kMaxColours; ++b) max_colour = std::max(max_colour, d.host_u32[3 + b] - d.host_u32[2 + b]);
     timing_mark(c, "colour");
     HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr));
     timing_mark(c, "gather");
   }
-  HIPC(launch_solve(c, C, p));
+  HIPC(launch_solve(c, C, max_colour, p));
   timing_mark(c, "solve");
   HIPC(launch_finish(c, C, p));
   timing_mark(c, "finish");
@@ -380,17 +397,23 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     if ((e = dalloc(&d.px, 3 * (size_t)c->cap_planes)) != hipSuccess ||
         (e = dalloc(&d.pn, 3 * (size_t)c->cap_planes)) != hipSuccess ||
         (e = dalloc(&d.coloff, (size_t)kMaxColours + 2)) != hipSuccess ||
-        (e = dalloc(&d.grid, 1)) != hipSuccess || (e = dalloc(&d.ctr, 1)) != hipSuccess) {
+        (e = dalloc(&d.grid, 1)) != hipSuccess || (e = dalloc(&d.ctr, 1)) != hipSuccess ||
+        (e = dalloc(&d.barrier, kBarrierWords)) != hipSuccess ||
+        (e = hipMemset(d.barrier, 0, sizeof(uint32_t) * kBarrierWords)) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
       break;
     }
     if (hipHostMalloc((void**)&d.ctr_host, sizeof(StepCounters)) != hipSuccess ||
-        hipHostMalloc((void**)&d.host_u32, 64 * sizeof(uint32_t)) != hipSuccess) {
+        hipHostMalloc((void**)&d.host_u32, 128 * sizeof(uint32_t)) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipHostMalloc failed");
       break;
     }
-    c->gs_blocks = solver_max_blocks(c);
-    c->colour_blocks = colour_max_blocks(c);
+    // Persistent (cooperative) kernels: the grid barrier's cost grows with the number of
+    // workgroups, so they run one workgroup per CU unless SCX_PERSISTENT_PER_CU says more.
+    int per_cu = 1;
+    if (const char* env = getenv("SCX_PERSISTENT_PER_CU")) per_cu = std::min(4, std::max(1, atoi(env)));
+    c->gs_blocks = std::min(solver_max_blocks(c), per_cu * c->num_cus);
+    c->colour_blocks = std::min(colour_max_blocks(c), per_cu * c->num_cus);
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
       break;
@@ -596,7 +619,7 @@ int scx_potential_overlaps(scx_ctx* c, uint32_t n, const double* aabb_min, const
   HIPC(hipMemcpyAsync(d.aabb_max, aabb_max, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
   for (int attempt = 0; attempt < 8; ++attempt) {
     HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
-    HIPC(launch_grid_build(c, n));
+    HIPC(launch_grid_build(c, n, true, false));
     HIPC(launch_pairs(c, n, 0, 0, nullptr, nullptr, 0));
     HIPC(launch_exclusive_scan(d.pcount, d.pcount, n, d.scan_tmp, s));
     HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));

@@ -7,12 +7,8 @@
 // 3D tangent choice (a branch on |n x u|^2 < 1e-9) rounds like the CPU restatement.
 #pragma clang fp contract(off)
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "scx_device.h"
 #include "scx_internal.h"
-
-namespace cg = cooperative_groups;
 
 namespace scx {
 
@@ -257,12 +253,11 @@ __global__ void __launch_bounds__(kBlock) k_adj_fill(uint32_t C, const uint32_t*
 __global__ void __launch_bounds__(kBlock) k_colour(uint32_t C, const uint32_t* __restrict__ ca,
                                                    const uint32_t* __restrict__ cb, const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ adj, int32_t* state,
-                                                   StepCounters* ctr) {
-  cg::grid_group grid = cg::this_grid();
+                                                   StepCounters* ctr, uint32_t* bar) {
   const uint32_t gsize = gridDim.x * blockDim.x;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t k = gtid; k < C; k += gsize) state[k] = -1;
-  grid.sync();
+  grid_barrier(bar);
   uint32_t done = 0;
   for (uint32_t round = 0; round <= C && done < C; ++round) {
     uint32_t mine = 0;
@@ -304,12 +299,12 @@ __global__ void __launch_bounds__(kBlock) k_colour(uint32_t C, const uint32_t* _
     mine = block_reduce_sum_u32(mine);
     if (threadIdx.x == 0 && mine) atomicAdd(&ctr->remaining, mine);
     if (maxcol) atomicMax(&ctr->ncolours, maxcol);
-    grid.sync();
+    grid_barrier(bar);
     done = __atomic_load_n(&ctr->remaining, __ATOMIC_RELAXED);
     if (gtid == 0 && done >= C) ctr->nrounds = round + 1;
     // every workgroup has read the count before any adds to it again, so all of them
     // take the same exit decision
-    grid.sync();
+    grid_barrier(bar);
   }
 }
 
@@ -342,7 +337,8 @@ hipError_t launch_colouring(scx_ctx* c, uint32_t C) {
   const uint32_t* padj = c->d.adj;
   int32_t* pstate = c->d.cstate;
   StepCounters* pctr = c->d.ctr;
-  void* args[] = {&Cv, &pa, &pb, &poff, &padj, &pstate, &pctr};
+  uint32_t* pbar = c->d.barrier;
+  void* args[] = {&Cv, &pa, &pb, &poff, &padj, &pstate, &pctr, &pbar};
   return hipLaunchCooperativeKernel((const void*)k_colour, dim3(blocks), dim3(kBlock), args, 0, s);
 }
 

@@ -63,23 +63,46 @@ __device__ __forceinline__ double wave_reduce_sum_f64(double v) {
   return v;
 }
 
-// Cell coordinates packed 20 bits per axis (60 bits); bits 60..62 carry, for a grid
-// entry, whether the cell is the first cell of its box along x / y / z.
-constexpr int kKeyBits = 20;
-constexpr uint32_t kKeyMax = (1u << kKeyBits) - 1;
-constexpr unsigned long long kKeyMask = (1ull << 60) - 1;
+// Grid-wide barrier for persistent kernels (launched cooperatively, so that every
+// workgroup is resident). Two levels: workgroups arrive on their group's counter
+// (groups of kBarrierGroup, one 128-byte line each); the last of a group arrives on the
+// top counter; the last group bumps the generation word that everyone else polls.
+// Counters return to zero, so the state is reusable without a reset. Every thread
+// fences at agent scope before arriving and after leaving, which publishes its
+// stores to, and drops stale lines from, the other CUs and XCDs.
+// Layout (u32 words): [0] generation, [32] top counter, [64 + 32 g] group g counter.
+constexpr uint32_t kBarrierGroup = 16;
+constexpr uint32_t kBarrierMaxGroups = 64;
+constexpr uint32_t kBarrierWords = 64 + 32 * kBarrierMaxGroups;
 
-__device__ __forceinline__ unsigned long long pack_cell(uint32_t x, uint32_t y, uint32_t z) {
-  return (unsigned long long)x | ((unsigned long long)y << 20) | ((unsigned long long)z << 40);
-}
-
-__device__ __forceinline__ uint32_t hash_cell(unsigned long long k) {
-  k ^= k >> 33;
-  k *= 0xff51afd7ed558ccdull;
-  k ^= k >> 33;
-  k *= 0xc4ceb9fe1a85ec53ull;
-  k ^= k >> 33;
-  return (uint32_t)k;
+__device__ __forceinline__ void grid_barrier(uint32_t* bar) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nblocks = gridDim.x;
+    const uint32_t group = blockIdx.x / kBarrierGroup;
+    const uint32_t ngroups = (nblocks + kBarrierGroup - 1) / kBarrierGroup;
+    const uint32_t gsize = min(kBarrierGroup, nblocks - group * kBarrierGroup);
+    uint32_t* gen = bar;
+    uint32_t* top = bar + 32;
+    uint32_t* gcnt = bar + 64 + 32 * group;
+    const uint32_t g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    bool last = false;
+    if (__hip_atomic_fetch_add(gcnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
+      __hip_atomic_store(gcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1) {
+        __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = true;
+      }
+    }
+    if (last) {
+      __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  __threadfence();
 }
 
 // Contact priority for the colouring rounds: (fmix32(c * golden + salt) << 32) | c.
@@ -96,11 +119,12 @@ __device__ __forceinline__ unsigned long long prio_key(uint32_t c) {
 }
 
 // Cell index of a coordinate: floor((x - min) / h) as the reference's unsigned cast
-// (ball2d/SpatialGridDetector.cpp:29-34), clamped to the packed key range.
-__device__ __forceinline__ uint32_t cell_of(double x, double mn, double h) {
+// (ball2d/SpatialGridDetector.cpp:29-34), clamped to [0, dim - 1] (monotone in x, which
+// is all the emission rule of the sweep needs).
+__device__ __forceinline__ uint32_t cell_of(double x, double mn, double h, uint32_t dim) {
   const double t = (x - mn) / h;
   if (!(t > 0.0)) return 0u;
-  if (!(t < (double)kKeyMax)) return kKeyMax;
+  if (!(t < (double)(dim - 1))) return dim - 1;
   return (uint32_t)t;
 }
 

@@ -24,8 +24,8 @@ constexpr int kMaxColours = 64;
 struct GridParams {
   double min_coord[3];
   double h;
-  uint32_t table_mask;   // hash table size - 1
-  uint32_t pad;
+  uint32_t dims[3];      // cells per axis (dense row-major grid)
+  uint32_t ncells;
 };
 
 // Device-side counters of one step, read back by the host at the sync points.
@@ -86,9 +86,14 @@ struct DeviceBuffers {
   double* aabb_max = nullptr;
   double* red = nullptr;       // reduction partials
   GridParams* grid = nullptr;
-  uint32_t* bucket = nullptr;  // table_size + 1 counts / starts
-  unsigned long long* ekey = nullptr;  // entries: packed cell key | first-cell flags
-  uint32_t* eid = nullptr;
+  uint32_t* cells = nullptr;   // 4 x (cap_cells + 1): entry counts, entry starts, home counts, home starts
+  uint32_t* ent = nullptr;     // cell entries: sorted body index | first-cell flags
+  uint32_t* sid = nullptr;     // home-cell order -> body id
+  double* smin = nullptr;      // boxes in home-cell order (dim*N)
+  double* smax = nullptr;
+  double* sq0 = nullptr;       // narrow-phase data in home-cell order
+  double* sq1 = nullptr;
+  double* sr = nullptr;
   uint32_t* pcount = nullptr;  // N + 1 per-owner counts / offsets
   uint32_t* scan_tmp = nullptr; // scratch of the device-wide scans
   uint32_t* pa = nullptr;      // pair lists of scx_potential_overlaps (capacity cap_pairs)
@@ -126,6 +131,7 @@ struct DeviceBuffers {
   uint32_t prev_count = 0;
   unsigned long long* cur_key = nullptr;
   StepCounters* ctr = nullptr;
+  uint32_t* barrier = nullptr;       // grid barrier state of the persistent kernels (zeroed once)
   StepCounters* ctr_host = nullptr;  // pinned
   uint32_t* host_u32 = nullptr;      // pinned scratch
 };
@@ -138,7 +144,7 @@ struct scx_ctx {
   int num_cus = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  uint32_t cap_bodies = 0, cap_planes = 0, cap_contacts = 0, cap_entries = 0, cap_pairs = 0, table_size = 0;
+  uint32_t cap_bodies = 0, cap_planes = 0, cap_contacts = 0, cap_entries = 0, cap_pairs = 0, cap_cells = 0;
   uint32_t cap_perm = 0;  // perm buffer length (plane compaction and colour order)
   uint32_t n = 0, nplanes = 0;
   bool has_bodies = false;
@@ -162,7 +168,7 @@ hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, c
                                const double* g, int integrator, double* q1p);
 hipError_t launch_flow(scx_ctx* c, uint32_t n, int integrator, const double* m, const double* g, double dt,
                        const double* q0, const double* v0, double* q1, double* v1);
-hipError_t launch_grid_build(scx_ctx* c, uint32_t n);
+hipError_t launch_grid_build(scx_ctx* c, uint32_t n, bool given_boxes, bool narrow_data);
 hipError_t launch_pairs(scx_ctx* c, uint32_t n, int mode, int filter, uint32_t* out_a, uint32_t* out_b, uint32_t cap);
 hipError_t launch_plane_masks(scx_ctx* c, uint32_t n);
 hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint32_t n, uint32_t nbuckets,
@@ -171,7 +177,7 @@ hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double C
 hipError_t launch_colouring(scx_ctx* c, uint32_t C);
 hipError_t launch_colour_sort(scx_ctx* c, uint32_t C);
 hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* rinit);
-hipError_t launch_solve(scx_ctx* c, uint32_t C, const scx_step_params* p);
+hipError_t launch_solve(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p);
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p);
 int solver_max_blocks(scx_ctx* c);
 int colour_max_blocks(scx_ctx* c);

@@ -14,15 +14,13 @@
 // H v + w and no Delassus off-diagonal block is ever formed.
 #pragma clang fp contract(fast)
 
-#include <hip/hip_cooperative_groups.h>
-
 #include "scx_device.h"
 #include "scx_flow.h"
 #include "scx_internal.h"
 
-namespace cg = cooperative_groups;
-
 namespace scx {
+
+constexpr int kGSBlock = 512;  // persistent sweep workgroup: 8 wave64s
 
 template <typename Real>
 struct Vec4 {
@@ -173,6 +171,7 @@ struct GSArgs {
   Real* rbest;              // 3 per contact
   double* partial;          // per-workgroup residual partials
   StepCounters* ctr;
+  uint32_t* bar;             // grid barrier state
   uint32_t max_iters;
   uint32_t eval_every;
   double tol;
@@ -265,8 +264,8 @@ __device__ __forceinline__ double contact_err(const GSArgs<Real>& A, uint32_t k)
 
 // Grid-wide max of the residual; identical in every thread after the barrier.
 template <int D, typename Real>
-__device__ double eval_residual(const GSArgs<Real>& A, cg::grid_group& grid, uint32_t gtid, uint32_t gsize) {
-  __shared__ double s_red[kBlock / 64];
+__device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize) {
+  __shared__ double s_red[kGSBlock / 64];
   double m = 0.0;
   for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real>(A, k));
   m = wave_reduce_max_f64(m);
@@ -274,18 +273,18 @@ __device__ double eval_residual(const GSArgs<Real>& A, cg::grid_group& grid, uin
   __syncthreads();
   if (threadIdx.x == 0) {
     double b = s_red[0];
-    for (int w = 1; w < kBlock / 64; ++w) b = fmax(b, s_red[w]);
+    for (int w = 1; w < kGSBlock / 64; ++w) b = fmax(b, s_red[w]);
     A.partial[blockIdx.x] = b;
   }
-  grid.sync();
+  grid_barrier(A.bar);
   double e = 0.0;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) e = fmax(e, A.partial[b]);
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kGSBlock) e = fmax(e, A.partial[b]);
   e = wave_reduce_max_f64(e);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = e;
   __syncthreads();
   double out = s_red[0];
-  for (int w = 1; w < kBlock / 64; ++w) out = fmax(out, s_red[w]);
+  for (int w = 1; w < kGSBlock / 64; ++w) out = fmax(out, s_red[w]);
   __syncthreads();
   return out;
 }
@@ -294,7 +293,7 @@ __device__ double eval_residual(const GSArgs<Real>& A, cg::grid_group& grid, uin
 // touched by a contact can differ from v0 (every body starts at v0, k_body_init), so
 // a reset touches those bodies only; two contacts sharing a body write the same v0.
 template <int D, typename Real>
-__device__ void apply_impulses(const GSArgs<Real>& A, cg::grid_group& grid, const uint32_t* off, uint32_t ncol,
+__device__ void apply_impulses(const GSArgs<Real>& A, const uint32_t* off, uint32_t ncol,
                                uint32_t gtid, uint32_t gsize, bool reset) {
   if (reset) {
     for (uint32_t k = gtid; k < A.C; k += gsize) {
@@ -309,22 +308,21 @@ __device__ void apply_impulses(const GSArgs<Real>& A, cg::grid_group& grid, cons
           for (int d = 0; d < 3; ++d) A.bw[4 * (size_t)i + d] = (Real)A.w0[3 * (size_t)i + d];
       }
     }
-    grid.sync();
+    grid_barrier(A.bar);
   }
   for (uint32_t col = 0; col < ncol; ++col) {
     for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, true);
-    grid.sync();
+    grid_barrier(A.bar);
   }
 }
 
 template <int D, typename Real>
-__global__ void __launch_bounds__(kBlock) k_gs(GSArgs<Real> A) {
-  cg::grid_group grid = cg::this_grid();
-  const uint32_t gsize = gridDim.x * kBlock;
-  const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+__global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
+  const uint32_t gsize = gridDim.x * kGSBlock;
+  const uint32_t gtid = blockIdx.x * kGSBlock + threadIdx.x;
   __shared__ uint32_t off[kMaxColours + 1];
   const uint32_t ncol = A.ctr->ncolours;
-  for (uint32_t c = threadIdx.x; c <= ncol; c += kBlock) off[c] = A.coloff[c];
+  for (uint32_t c = threadIdx.x; c <= ncol; c += kGSBlock) off[c] = A.coloff[c];
   __syncthreads();
   uint32_t max_iters = A.max_iters;
   // FP32 phase: at most half the sweep budget, the FP64 polish gets the rest
@@ -332,15 +330,15 @@ __global__ void __launch_bounds__(kBlock) k_gs(GSArgs<Real> A) {
   if (A.phase == 2) max_iters = (A.ctr->sweeps_fp32 < max_iters) ? max_iters - A.ctr->sweeps_fp32 : 0u;
   const bool track_best = A.phase != 1;
 
-  if (A.apply_init) apply_impulses<D, Real>(A, grid, off, ncol, gtid, gsize, false);
+  if (A.apply_init) apply_impulses<D, Real>(A, off, ncol, gtid, gsize, false);
 
-  double err = eval_residual<D, Real>(A, grid, gtid, gsize);
+  double err = eval_residual<D, Real>(A, gtid, gsize);
   double best = err;
   if (track_best) {
     for (uint32_t k = gtid; k < A.C; k += gsize)
 #pragma unroll
       for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
-    grid.sync();
+    grid_barrier(A.bar);
   }
   // FP32 phase stagnation: leave when 64 sweeps did not halve the best residual
   double ref = best;
@@ -351,17 +349,17 @@ __global__ void __launch_bounds__(kBlock) k_gs(GSArgs<Real> A) {
     for (it = 1; it <= max_iters; ++it) {
       for (uint32_t col = 0; col < ncol; ++col) {
         for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false);
-        grid.sync();
+        grid_barrier(A.bar);
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
-        err = eval_residual<D, Real>(A, grid, gtid, gsize);
+        err = eval_residual<D, Real>(A, gtid, gsize);
         if (err < best) {
           best = err;
           if (track_best) {
             for (uint32_t k = gtid; k < A.C; k += gsize)
 #pragma unroll
               for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
-            grid.sync();
+            grid_barrier(A.bar);
           }
         }
         if (err < A.tol) break;
@@ -380,8 +378,8 @@ __global__ void __launch_bounds__(kBlock) k_gs(GSArgs<Real> A) {
       for (uint32_t k = gtid; k < A.C; k += gsize)
 #pragma unroll
         for (int d = 0; d < 3; ++d) A.c.r[3 * (size_t)k + d] = A.rbest[3 * (size_t)k + d];
-      grid.sync();
-      apply_impulses<D, Real>(A, grid, off, ncol, gtid, gsize, true);
+      grid_barrier(A.bar);
+      apply_impulses<D, Real>(A, off, ncol, gtid, gsize, true);
     }
     if (it > max_iters) it = max_iters;
   }
@@ -467,25 +465,28 @@ __global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, co
   for (int d = 1; d < dim; ++d) beta[(size_t)(dim - 1) * c + d - 1] = (mu == 0.0) ? 0.0 : r64[3 * (size_t)k + d];
 }
 
+// Persistent launch sized to the largest colour: one contact per thread per colour
+// pass, so the barrier (whose cost grows with the number of workgroups) spans no
+// more workgroups than the widest pass needs.
 template <int D, typename Real>
-static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks) {
-  const uint32_t want = (std::max<uint32_t>(A.C, 1u) + kBlock - 1) / kBlock;
+static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, uint32_t max_colour) {
+  const uint32_t want = (std::max<uint32_t>(max_colour, 1u) + kGSBlock - 1) / kGSBlock;
   const int blocks = (int)std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)max_blocks, want));
   void* args[] = {&A};
-  return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kBlock), args, 0, c->stream);
+  return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
 }
 
 template <int D, typename Real>
 int gs_max_blocks(scx_ctx* c) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gs<D, Real>, kBlock, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gs<D, Real>, kGSBlock, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   return per_cu * c->num_cus;
 }
 
 template <int D>
-static hipError_t solve_dim(scx_ctx* c, uint32_t C, const scx_step_params* p) {
+static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p) {
   hipStream_t s = c->stream;
   const bool fp32 = p->precision == SCX_PRECISION_FP32_POLISH;
   const bool warm = p->warm_start != SCX_WARM_NONE;
@@ -497,9 +498,9 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const scx_step_params* p) {
   if (e != hipSuccess || C == 0) return e;
   if (fp32) {
     GSArgs<float> A{c->d.c32, c->d.bv32, c->d.bw32, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
-                    reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, p->max_iters,
+                    reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
-    e = coop_gs<D, float>(c, A, c->gs_blocks);
+    e = coop_gs<D, float>(c, A, c->gs_blocks, max_colour);
     if (e != hipSuccess) return e;
     const uint32_t nr = (3 * C + kBlock - 1) / kBlock;
     k_promote_r<<<nr, kBlock, 0, s>>>(C, c->d.c32.r, c->d.c64.r);
@@ -507,12 +508,13 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const scx_step_params* p) {
     if (e != hipSuccess) return e;
   }
   GSArgs<double> B{c->d.c64, c->d.bv64, c->d.bw64, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
-                   c->d.partial, c->d.ctr, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0, (fp32 || warm) ? 1 : 0};
-  return coop_gs<D, double>(c, B, c->gs_blocks);
+                   c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
+                   (fp32 || warm) ? 1 : 0};
+  return coop_gs<D, double>(c, B, c->gs_blocks, max_colour);
 }
 
-hipError_t launch_solve(scx_ctx* c, uint32_t C, const scx_step_params* p) {
-  return c->dim == 2 ? solve_dim<2>(c, C, p) : solve_dim<3>(c, C, p);
+hipError_t launch_solve(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p) {
+  return c->dim == 2 ? solve_dim<2>(c, C, max_colour, p) : solve_dim<3>(c, C, max_colour, p);
 }
 
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {

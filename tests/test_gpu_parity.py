@@ -183,13 +183,15 @@ def test_fp32_sweeps_with_fp64_polish(ctx2, ctx3, dim):
     if dim == 2:
         s = scenes.ball2d_lattice(m=48, side=7.5, r=0.05, vstd=0.1, seed=3)
         sim = oracle.Sim(2, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"])
-        ctx, prm = ctx2, _params(2, dt=0.01, mu=0.5, tol=1e-8, max_iters=20000)
-        sim.run(60, **prm)
+        # a loose pile, where Gauss-Seidel reaches a tight tolerance (jammed piles stall,
+        # in every contact order, as projected GS does on frictional networks)
+        ctx, prm = ctx2, _params(2, dt=0.01, mu=0.5, tol=1e-10, max_iters=20000)
+        sim.run(25, **prm)
     else:
         s = _spheres_in_box(3000, 4)
         sim = oracle.Sim(3, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"], inertia=s["inertia"])
         ctx, prm = ctx3, _params(3, dt=0.005, mu=0.4, tol=1e-10, max_iters=20000)
-        sim.run(80, **prm)
+        sim.run(40, **prm)
     ctx.set_planes(s["px"], s["pn"])
     ctx.set_bodies(sim.q, sim.v, sim.r, sim.m, omega=sim.w, inertia=sim.inertia)
     st = ctx.step(scx.step_params(precision=scx.FP32_POLISH, **prm))
