@@ -129,6 +129,37 @@ int scx_get_contacts(scx_ctx* ctx, uint32_t cap, int32_t* body0, int32_t* body1,
 /* Last step's post-impact velocity v2 (dim*n; 3D: plus omega2 3n if not NULL). */
 int scx_get_v2(scx_ctx* ctx, double* v2, double* omega2);
 
+/* An explicit active set for scx_friction_solve, in the reference's conventions
+ * (Sobogus::solve inputs, scisim/ConstrainedMaps/Sobogus.cpp:880-1088): body1 = -1 for
+ * static geometry; bases column-major [n s t] per contact (right-handed, orthonormal,
+ * as Constraint::computeBasis asserts, scisim/Constraints/Constraint.cpp:10-16);
+ * 3D arms = contact point - body centre at q0 for each body (sphere arms are parallel to n);
+ * nrel_extra is added to the normal forcing term; drel_extra must be NULL. */
+typedef struct scx_contact_set {
+  uint32_t ncontacts;
+  const int32_t* body0;
+  const int32_t* body1;
+  const double* bases;
+  const double* arm0;      /* 3D: 3 per contact; NULL in 2D */
+  const double* arm1;      /* 3D: 3 per contact; NULL in 2D */
+  const double* mu;        /* per contact */
+  const double* CoR;       /* per contact, NULL = 0 */
+  const double* nrel_extra;
+  const double* drel_extra;
+} scx_contact_set;
+
+/* FrictionSolver::solve (scisim/ConstrainedMaps/FrictionSolver.h:28; Sobogus::solve,
+ * scisim/ConstrainedMaps/Sobogus.cpp:880-1088) on an explicit active set: masses m (and
+ * 3D isotropic inertia), start velocities v0 (dim per body; 3D omega0 3 per body). alpha
+ * (ncontacts) and beta ((dim-1) per contact) carry the warm start in and the impulses out.
+ * Outputs: v2 (and omega2) = v0 + M^-1 H^T r; f = friction-only generalized impulse
+ * ([lin | ang] per body in 3D); succeeded = error < tol; error = max ||FB||^2; sweeps.
+ * Replaces any resident scene (call scx_set_bodies again before stepping). */
+int scx_friction_solve(scx_ctx* ctx, uint32_t nbodies, const double* m, const double* inertia, const double* v0,
+                       const double* omega0, const scx_contact_set* contacts, double tol, uint32_t max_iters,
+                       uint32_t eval_every, int32_t precision, double* alpha, double* beta, double* v2,
+                       double* omega2, double* f, int32_t* succeeded, double* error, uint32_t* sweeps);
+
 /* UnconstrainedMap::flow on host arrays (q0, v0 -> q1, v1), gravity only. */
 int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const double* m, const double* g, double dt,
                            const double* q0, const double* v0, double* q1, double* v1);

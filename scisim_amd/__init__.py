@@ -31,7 +31,7 @@ WARM_CACHE = 1
 # Every symbol include/scx.h declares.
 EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_potential_overlaps", "scx_set_bodies",
            "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
-           "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing"]
+           "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing"]
 
 _d = ctypes.c_double
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -59,6 +59,11 @@ class ScxStepStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class ScxContactSet(ctypes.Structure):
+    _fields_ = [("ncontacts", ctypes.c_uint32), ("body0", _pi32), ("body1", _pi32), ("bases", _pd), ("arm0", _pd),
+                ("arm1", _pd), ("mu", _pd), ("CoR", _pd), ("nrel_extra", _pd), ("drel_extra", _pd)]
 
 
 _lib = None
@@ -89,6 +94,9 @@ def lib():
         L.scx_run.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.c_uint32, ctypes.POINTER(ScxStepStats)]
         L.scx_get_contacts.argtypes = [_vp, ctypes.c_uint32, _pi32, _pi32, _pi32, _pd, _pd, _pd, _pi32, _pu32]
         L.scx_get_v2.argtypes = [_vp, _pd, _pd]
+        L.scx_friction_solve.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, _pd, _pd, ctypes.POINTER(ScxContactSet), _d,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, _pd, _pd, _pd, _pd, _pd,
+                                         _pi32, _pd, _pu32]
         L.scx_unconstrained_flow.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint32, _pd, _pd, _d, _pd, _pd, _pd, _pd]
         L.scx_set_timing.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_timing.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p), _pd, _pu32]
@@ -213,6 +221,34 @@ class Context:
         w2 = np.empty(3 * self.n) if self.dim == 3 else None
         self._check(lib().scx_get_v2(self._h, _p(v2), _p(w2)), "scx_get_v2")
         return v2, w2
+
+    def friction_solve(self, m, v0, body0, body1, bases, mu, inertia=None, omega0=None, arm0=None, arm1=None,
+                       CoR=None, alpha=None, beta=None, tol=1e-12, max_iters=5000, eval_every=1, precision=FP64):
+        """FrictionSolver::solve on an explicit active set; returns a dict of outputs."""
+        D = self.dim
+        m, v0 = _f64(m), _f64(v0)
+        n = m.size
+        C = len(body0)
+        b0 = np.ascontiguousarray(body0, np.int32)
+        b1 = np.ascontiguousarray(body1, np.int32)
+        bases, mu = _f64(bases), _f64(mu)
+        arm0, arm1, CoR = _f64(arm0), _f64(arm1), _f64(CoR)
+        cs = ScxContactSet(ncontacts=C, body0=_p(b0, _pi32), body1=_p(b1, _pi32), bases=_p(bases), arm0=_p(arm0),
+                           arm1=_p(arm1), mu=_p(mu), CoR=_p(CoR), nrel_extra=None, drel_extra=None)
+        al = np.zeros(C) if alpha is None else _f64(alpha).copy()
+        be = np.zeros((D - 1) * C) if beta is None else _f64(beta).copy()
+        v2 = np.empty(D * n)
+        w2 = np.empty(3 * n) if D == 3 else None
+        f = np.empty((2 if D == 2 else 6) * n)
+        ok = ctypes.c_int32(0)
+        err = ctypes.c_double(0.0)
+        sw = ctypes.c_uint32(0)
+        self._check(lib().scx_friction_solve(self._h, n, _p(m), _p(_f64(inertia)), _p(v0), _p(_f64(omega0)),
+                                             ctypes.byref(cs), tol, max_iters, eval_every, precision, _p(al), _p(be),
+                                             _p(v2), _p(w2), _p(f), ctypes.byref(ok), ctypes.byref(err),
+                                             ctypes.byref(sw)), "scx_friction_solve")
+        return {"alpha": al, "beta": be.reshape(C, D - 1), "v2": v2, "omega2": w2, "f": f,
+                "succeeded": bool(ok.value), "error": err.value, "sweeps": sw.value}
 
     def unconstrained_flow(self, integrator, m, g, dt, q0, v0):
         m, q0, v0 = _f64(m), _f64(q0), _f64(v0)

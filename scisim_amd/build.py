@@ -53,9 +53,23 @@ def build_scenes():
     os.replace(tmp, SCENES_LIB)
 
 
+def build_cpp_tests():
+    """C++ programs written against include/scx_scisim.hpp (the reference-shaped host
+    interface), linked to libscx.so; run by tests/test_gpu_cpp_interface.py."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    for name in ("spatial_grid_tests", "slope_tests"):
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
+              os.path.join(src_dir, name + ".cpp"), "-o", os.path.join(out_dir, name), "-L", HERE, "-lscx",
+              "-Wl,-rpath," + HERE])
+
+
 def build():
     build_scx()
     build_scenes()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":

@@ -307,7 +307,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     HIPC(hipStreamSynchronize(s));
     for (int b = 0; b < kMaxColours; ++b) max_colour = std::max(max_colour, d.host_u32[3 + b] - d.host_u32[2 + b]);
     timing_mark(c, "colour");
-    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr));
+    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr, nullptr));
     timing_mark(c, "gather");
   }
   HIPC(launch_solve(c, C, max_colour, p));
@@ -596,6 +596,160 @@ int scx_get_v2(scx_ctx* c, double* v2, double* omega2) {
   HIPC(hipSetDevice(c->device));
   if (v2) HIPC(hipMemcpy(v2, c->d.v2, sizeof(double) * c->dim * c->n, hipMemcpyDeviceToHost));
   if (omega2 && c->dim == 3) HIPC(hipMemcpy(omega2, c->d.w2, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost));
+  return SCX_OK;
+}
+
+int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const double* inertia, const double* v0,
+                       const double* omega0, const scx_contact_set* cs, double tol, uint32_t max_iters,
+                       uint32_t eval_every, int32_t precision, double* alpha, double* beta, double* v2,
+                       double* omega2, double* f, int32_t* succeeded, double* error, uint32_t* sweeps) {
+  if (!c) return SCX_E_ARG;
+  if (!cs || !m || !v0 || !alpha || !beta) return fail(c, SCX_E_ARG, "scx_friction_solve: null argument");
+  if (precision != SCX_PRECISION_FP64 && precision != SCX_PRECISION_FP32_POLISH)
+    return fail(c, SCX_E_ARG, "unknown precision mode %d", precision);
+  HIPC(hipSetDevice(c->device));
+  const int D = c->dim;
+  const uint32_t C = cs->ncontacts;
+  if (nbodies == 0) return fail(c, SCX_E_ARG, "no bodies");
+  if (C > 0 && (!cs->body0 || !cs->body1 || !cs->bases || !cs->mu))
+    return fail(c, SCX_E_ARG, "scx_friction_solve: incomplete contact set");
+  if (D == 3 && C > 0 && (!cs->arm0 || !cs->arm1)) return fail(c, SCX_E_ARG, "3D contacts need arms");
+  if (cs->drel_extra) return fail(c, SCX_E_ARG, "tangential forcing terms (drel_extra) are not supported");
+  for (uint32_t k = 0; k < C; ++k) {
+    if (cs->body0[k] < 0 || (uint32_t)cs->body0[k] >= nbodies || cs->body1[k] < -1 ||
+        cs->body1[k] >= (int32_t)nbodies || cs->body0[k] == cs->body1[k])
+      return fail(c, SCX_E_ARG, "contact %u: invalid bodies (%d, %d)", k, cs->body0[k], cs->body1[k]);
+    if (!(cs->mu[k] >= 0.0)) return fail(c, SCX_E_ARG, "contact %u: mu must be non-negative", k);
+  }
+  if (nbodies > c->cap_bodies) {
+    if (alloc_bodies(c, nbodies) != SCX_OK || alloc_scan_tmp(c) != SCX_OK) return SCX_E_HIP;
+  }
+  if (C > c->cap_contacts) {
+    if (alloc_contacts(c, C) != SCX_OK || alloc_scan_tmp(c) != SCX_OK) return SCX_E_HIP;
+  }
+  // the explicit problem replaces the resident scene
+  c->has_bodies = false;
+  c->n = nbodies;
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  std::vector<double> Ivec(nbodies, 0.0), w0(3 * (size_t)nbodies, 0.0);
+  if (D == 3) {
+    for (uint32_t i = 0; i < nbodies; ++i) Ivec[i] = inertia ? inertia[i] : 0.0;
+    if (omega0) std::copy(omega0, omega0 + 3 * (size_t)nbodies, w0.begin());
+  }
+  // per-contact frames, arms, Delassus diagonal, forcing term and warm start (host; this
+  // entry point serves the reference's FrictionSolver boundary, not the step loop)
+  std::vector<uint32_t> ca(C), cb(C);
+  std::vector<double> la(C, 0.0), lb(C, 0.0), wn(C), wt(C), forcing(C), rinit((size_t)D * C), mu(cs->mu, cs->mu + C);
+  for (uint32_t k = 0; k < C; ++k) {
+    const uint32_t a = (uint32_t)cs->body0[k];
+    const int32_t b = cs->body1[k];
+    ca[k] = a;
+    cb[k] = b >= 0 ? (uint32_t)b : kNoBody;
+    const double* B = cs->bases + (size_t)D * D * k;
+    const double minv_a = 1.0 / m[a], minv_b = b >= 0 ? 1.0 / m[b] : 0.0;
+    double ndotv = 0.0;
+    for (int q = 0; q < D; ++q) ndotv += B[q] * (v0[(size_t)D * a + q] - (b >= 0 ? v0[(size_t)D * b + q] : 0.0));
+    const double cor = cs->CoR ? cs->CoR[k] : 0.0;
+    forcing[k] = cor * ndotv + (cs->nrel_extra ? cs->nrel_extra[k] : 0.0);
+    if (D == 2) {
+      wn[k] = minv_a + minv_b;
+      wt[k] = minv_a + minv_b;
+    } else {
+      const double* pa = cs->arm0 + 3 * (size_t)k;
+      const double* pb = cs->arm1 + 3 * (size_t)k;
+      la[k] = std::sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]);
+      lb[k] = b >= 0 ? std::sqrt(pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2]) : 0.0;
+      const double ia = 1.0 / Ivec[a], ib = b >= 0 ? 1.0 / Ivec[b] : 0.0;
+      wn[k] = minv_a + minv_b;
+      wt[k] = (minv_a + ia * la[k] * la[k]) + (b >= 0 ? (minv_b + ib * lb[k] * lb[k]) : 0.0);
+    }
+    rinit[(size_t)D * k] = alpha[k];
+    for (int q = 1; q < D; ++q) rinit[(size_t)D * k + q] = beta[(size_t)(D - 1) * k + q - 1];
+  }
+  bool warm = false;
+  for (double x : rinit) warm = warm || x != 0.0;
+  HIPC(hipMemcpyAsync(d.m, m, sizeof(double) * nbodies, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.inertia, Ivec.data(), sizeof(double) * nbodies, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.v0, v0, sizeof(double) * D * nbodies, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.w0, w0.data(), sizeof(double) * 3 * nbodies, hipMemcpyHostToDevice, s));
+  HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+  uint32_t max_colour = 0;
+  if (C > 0) {
+    HIPC(hipMemcpyAsync(d.ca, ca.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.cb, cb.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.basis, cs->bases, sizeof(double) * D * D * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.cla, la.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.clb, lb.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.cwn, wn.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.cwt, wt.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.cforcing, forcing.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.crinit, rinit.data(), sizeof(double) * D * C, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(d.calpha, mu.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));  // mu staging
+    HIPC(launch_colouring(c, C));
+    HIPC(launch_colour_sort(c, C));
+    HIPC(hipMemcpyAsync(d.host_u32 + 2, d.coloff, sizeof(uint32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    for (int b = 0; b < kMaxColours; ++b) max_colour = std::max(max_colour, d.host_u32[3 + b] - d.host_u32[2 + b]);
+    HIPC(launch_gather_records(c, C, precision == SCX_PRECISION_FP32_POLISH, 0.0, d.calpha, d.crinit));
+  }
+  scx_step_params p{};
+  p.dt = 1.0;
+  p.tol = tol;
+  p.max_iters = max_iters;
+  p.eval_every = eval_every;
+  p.precision = precision;
+  p.warm_start = warm ? SCX_WARM_CACHE : SCX_WARM_NONE;
+  p.mu = 1.0;  // beta is taken as solved (per-contact mu is in the records)
+  HIPC(launch_solve(c, C, max_colour, &p));
+  if (C > 0) {
+    const uint32_t nc = C;
+    (void)nc;
+    HIPC(launch_impulses(c, C));
+  }
+  std::vector<double> bv(4 * (size_t)nbodies), bw(D == 3 ? 4 * (size_t)nbodies : 0);
+  HIPC(hipMemcpyAsync(bv.data(), d.bv64, sizeof(double) * bv.size(), hipMemcpyDeviceToHost, s));
+  if (D == 3) HIPC(hipMemcpyAsync(bw.data(), d.bw64, sizeof(double) * bw.size(), hipMemcpyDeviceToHost, s));
+  if (C > 0) {
+    HIPC(hipMemcpyAsync(alpha, d.calpha, sizeof(double) * C, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(beta, d.cbeta, sizeof(double) * (D - 1) * C, hipMemcpyDeviceToHost, s));
+  }
+  HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (C > 0 && (d.ctr_host->overflow & 8u)) return fail(c, SCX_E_COLOURS, "contact graph needs more than 64 colours");
+  for (uint32_t i = 0; i < nbodies; ++i) {
+    if (v2)
+      for (int q = 0; q < D; ++q) v2[(size_t)D * i + q] = bv[4 * (size_t)i + q];
+    if (omega2 && D == 3)
+      for (int q = 0; q < 3; ++q) omega2[3 * (size_t)i + q] = bw[4 * (size_t)i + q];
+  }
+  // friction-only generalized impulse (Sobogus.cpp:335-346, :543-566): layout [lin | ang]
+  if (f) {
+    const size_t nv = (size_t)(D == 2 ? 2 : 6) * nbodies;
+    std::fill(f, f + nv, 0.0);
+    for (uint32_t k = 0; k < C; ++k) {
+      const double* B = cs->bases + (size_t)D * D * k;
+      double ft[3] = {0.0, 0.0, 0.0};
+      for (int col = 1; col < D; ++col)
+        for (int q = 0; q < D; ++q) ft[q] += beta[(size_t)(D - 1) * k + col - 1] * B[col * D + q];
+      const int32_t bodies[2] = {cs->body0[k], cs->body1[k]};
+      for (int side = 0; side < 2; ++side) {
+        const int32_t i = bodies[side];
+        if (i < 0) continue;
+        const double sg = side == 0 ? 1.0 : -1.0;
+        for (int q = 0; q < D; ++q) f[(size_t)D * i + q] += sg * ft[q];
+        if (D == 3) {
+          const double* r = (side == 0 ? cs->arm0 : cs->arm1) + 3 * (size_t)k;
+          const double tq[3] = {r[1] * ft[2] - r[2] * ft[1], r[2] * ft[0] - r[0] * ft[2], r[0] * ft[1] - r[1] * ft[0]};
+          for (int q = 0; q < 3; ++q) f[3 * (size_t)nbodies + 3 * (size_t)i + q] += sg * tq[q];
+        }
+      }
+    }
+  }
+  const double err = (C == 0) ? 0.0 : d.ctr_host->error;
+  if (error) *error = err;
+  if (succeeded) *succeeded = (C == 0 || err < tol) ? 1 : 0;
+  if (sweeps) *sweeps = d.ctr_host->sweeps;
   return SCX_OK;
 }
 

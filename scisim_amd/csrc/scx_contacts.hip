@@ -364,7 +364,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
                                                    const double* __restrict__ basis, const double* __restrict__ la,
                                                    const double* __restrict__ lb, const double* __restrict__ wn,
                                                    const double* __restrict__ wt, const double* __restrict__ forcing,
-                                                   const double* __restrict__ rinit, double mu, ContactSoA<double> o64,
+                                                   const double* __restrict__ rinit, double mu, const double* __restrict__ mu_arr,
+                                                   ContactSoA<double> o64,
                                                    ContactSoA<float> o32, int fp32) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= C) return;
@@ -387,7 +388,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
   o64.lb[k] = lb[c];
   o64.wn[k] = wn[c];
   o64.wt[k] = wt[c];
-  o64.mu[k] = mu;
+  const double muc = mu_arr ? mu_arr[c] : mu;
+  o64.mu[k] = muc;
   o64.forcing[k] = forcing[c];
   if (fp32) {
     o32.a[k] = ca[c];
@@ -401,16 +403,17 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
     o32.lb[k] = (float)lb[c];
     o32.wn[k] = (float)wn[c];
     o32.wt[k] = (float)wt[c];
-    o32.mu[k] = (float)mu;
+    o32.mu[k] = (float)muc;
     o32.forcing[k] = (float)forcing[c];
   }
 }
 
-hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* rinit) {
+hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* mu_arr,
+                                 const double* rinit) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
   k_gather<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, c->d.perm, c->d.ca, c->d.cb, c->d.basis, c->d.cla, c->d.clb,
-                                           c->d.cwn, c->d.cwt, c->d.cforcing, rinit, mu, c->d.c64, c->d.c32,
+                                           c->d.cwn, c->d.cwt, c->d.cforcing, rinit, mu, mu_arr, c->d.c64, c->d.c32,
                                            fp32 ? 1 : 0);
   return hipGetLastError();
 }

@@ -176,9 +176,11 @@ hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint3
 hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double mu);
 hipError_t launch_colouring(scx_ctx* c, uint32_t C);
 hipError_t launch_colour_sort(scx_ctx* c, uint32_t C);
-hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* rinit);
+hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* mu_arr,
+                                 const double* rinit);
 hipError_t launch_solve(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p);
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p);
+hipError_t launch_impulses(scx_ctx* c, uint32_t C);
 int solver_max_blocks(scx_ctx* c);
 int colour_max_blocks(scx_ctx* c);
 }  // namespace scx

@@ -533,6 +533,12 @@ hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {
   return hipGetLastError();
 }
 
+hipError_t launch_impulses(scx_ctx* c, uint32_t C) {
+  const uint32_t nc = (C + kBlock - 1) / kBlock;
+  if (nc) k_impulses_out<<<nc, kBlock, 0, c->stream>>>(C, c->dim, c->d.perm, c->d.c64.r, c->d.calpha, c->d.cbeta, 1.0);
+  return hipGetLastError();
+}
+
 int solver_max_blocks(scx_ctx* c) {
   return c->dim == 2 ? std::min(gs_max_blocks<2, double>(c), gs_max_blocks<2, float>(c))
                      : std::min(gs_max_blocks<3, double>(c), gs_max_blocks<3, float>(c));
