@@ -12,8 +12,8 @@ Workloads (synthetic, seeded):
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
 narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final
 update), with the state resident in HBM. value = bodies * steps / wall time of the
-timed steps. The dominant kernel's roofline uses algorithmic bytes per launch (see
-gs_bytes) over its average duration from HIP events on the library's stream. The CPU
+timed steps. The dominant kernel's roofline uses algorithmic bytes per launch
+(scisim_amd/roofline.py) over its average duration from HIP events on the library's stream. The CPU
 baseline is the FP64 oracle (a restatement of the reference path) timed on a bounded
 prefix of the same workload on the host.
 """
@@ -28,28 +28,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-
-HBM_PEAK_GBS = 8000.0
-
-
-def gs_sweep_bytes(dim, fp32):
-    """Algorithmic bytes per contact per Gauss-Seidel sweep: contact record read, impulse
-    read + write, both bodies' velocity (dof components) read + write."""
-    w = 4 if fp32 else 8
-    dof = 2 if dim == 2 else 6
-    if dim == 2:
-        rec = 8 + 2 * w + w + w + w     # ids, n, W, mu, forcing
-    else:
-        rec = 8 + 6 * w + 2 * w + 2 * w + w + w  # ids, n, s, arms, W_n, W_t, mu, forcing
-    return rec + 2 * dim * w + 2 * 2 * dof * w
-
-
-def gs_eval_bytes(dim, fp32):
-    """Residual evaluation per contact: record and impulse read, both bodies read."""
-    w = 4 if fp32 else 8
-    dof = 2 if dim == 2 else 6
-    return gs_sweep_bytes(dim, fp32) - dim * w - 2 * dof * w
-
 
 def make_scene(cfg, seed):
     from scisim_amd import scenes
@@ -105,6 +83,7 @@ def main():
         sys.exit(2)
 
     import scisim_amd as scx
+    from scisim_amd import roofline as rl
 
     def log(*a):
         if args.verbose:
@@ -135,20 +114,22 @@ def main():
     # timed region: every step synchronises at its end; per-phase device times from HIP events
     ctx.set_timing(True)
     phase_ms = {}
-    gs_bytes = 0.0
+    pairs_bytes = gs_bytes = 0.0
     contacts, sweeps = [], []
+    fp32 = precision == scx.FP32_POLISH
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = ctx.step(params)
         for name, ms in ctx.timing():
             phase_ms[name] = phase_ms.get(name, 0.0) + ms
         C = st["ncontacts"]
-        fp32 = precision == scx.FP32_POLISH
+        for write_pass in (False, True):
+            pairs_bytes += rl.pairs_launch_bytes(dim, n, st["ncells"], st["nentries"], st["npairs"], write_pass)
         s32 = st["sweeps_fp32"] if fp32 else 0
         s64 = st["sweeps"] - s32
         if fp32:
-            gs_bytes += C * (s32 * gs_sweep_bytes(dim, True) + (s32 + 1) * gs_eval_bytes(dim, True))
-        gs_bytes += C * (s64 * gs_sweep_bytes(dim, False) + (s64 + 1) * gs_eval_bytes(dim, False))
+            gs_bytes += rl.gs_launch_bytes(dim, C, s32, s32 + 1, True)
+        gs_bytes += rl.gs_launch_bytes(dim, C, s64, s64 + 1, False)
         contacts.append(C)
         sweeps.append(st["sweeps"])
         log("step", k, st)
@@ -156,8 +137,15 @@ def main():
     ctx.set_timing(False)
     ms_per_step = 1000.0 * wall / args.steps
     value = n * args.steps / wall
-    gs_ms = phase_ms.get("solve", 0.0)
-    achieved = gs_bytes / (gs_ms * 1e-3) / 1e9 if gs_ms > 0 else 0.0
+    # dominant kernel: the neighbour sweep (two launches per step) or the Gauss-Seidel
+    # sweeps (one FP32 and one FP64 launch per step), by device time
+    pairs_ms = phase_ms.get("pairs_count", 0.0) + phase_ms.get("pairs_write", 0.0)
+    gs_ms = phase_ms.get("gs_fp32", 0.0) + phase_ms.get("gs_fp64", 0.0)
+    if pairs_ms >= gs_ms:
+        kernel, launches, kbytes, kms = "k_pairs (grid neighbour sweep + FP64 narrow phase)", 2, pairs_bytes, pairs_ms
+    else:
+        kernel, launches, kbytes, kms = "k_gs (coloured Gauss-Seidel, persistent)", 1 + int(fp32), gs_bytes, gs_ms
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     out = {
         "metric": "body-steps/sec at FB residual<=1e-6",
         "value": value,
@@ -169,12 +157,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 sweeps + f64 residual/polish" if precision == scx.FP32_POLISH else "f64",
+        "dtype": "f32 sweeps + f64 residual/polish" if fp32 else "f64",
         "data": "synthetic",
         "config": dict(desc, mean_contacts=float(np.mean(contacts)), mean_sweeps=float(np.mean(sweeps)),
                        phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()}),
-        "roofline": {"bound": "hbm", "kernel": "k_gs (coloured Gauss-Seidel, persistent)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": rl.HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / rl.HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kbytes / (launches * args.steps),
+                     "us_per_launch": 1000.0 * kms / (launches * args.steps)},
     }
     ctx.close()
     if not args.no_cpu:

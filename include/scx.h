@@ -89,7 +89,7 @@ typedef struct scx_step_stats {
   uint32_t sweeps_fp32;
   double error;         /* best residual, max ||FB||^2 */
   int32_t succeeded;    /* error < tol */
-  int32_t pad;
+  uint32_t ncells;      /* cells of the broad-phase grid */
 } scx_step_stats;
 
 const char* scx_version(void);

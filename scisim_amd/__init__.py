@@ -55,10 +55,10 @@ class ScxStepStats(ctypes.Structure):
     _fields_ = [("nentries", ctypes.c_uint64), ("npairs", ctypes.c_uint32), ("nplane_contacts", ctypes.c_uint32),
                 ("ncontacts", ctypes.c_uint32), ("ncolours", ctypes.c_uint32), ("sweeps", ctypes.c_uint32),
                 ("sweeps_fp32", ctypes.c_uint32), ("error", _d), ("succeeded", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("ncells", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ScxContactSet(ctypes.Structure):

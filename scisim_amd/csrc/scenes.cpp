@@ -183,4 +183,153 @@ int scn_rsa3d(uint32_t n, double rmed, double sigma, double rclamp_lo, double rc
   return rsa<3>(n, radii, lo, hi, seed, q, r, max_attempts);
 }
 
+// A poured pile: spheres with radii rmed * exp(sigma Z) (clamped) are dropped one at a
+// time, in index order, at uniform (x, z) over the base [-base/2, base/2]^2 onto the
+// floor y = 0 and the spheres already placed, and roll downhill until they rest.
+// Rolling is a projected descent: a trial move straight down is pushed back out of every
+// sphere it enters (and kept inside the walls and above the floor); the move is kept if
+// it lowers the sphere without overlap, otherwise the step is halved, until the step is
+// below 1e-7 r. A sphere so comes to rest in a pocket of (typically) three supports,
+// which gives a pile with about six contacts per sphere. Each sphere is finally lowered
+// by `sink` so that it overlaps its supports slightly and its contacts are active at the
+// first step. Writes q (3n), r (n) and the pile height.
+int scn_deposit3d(uint32_t n, double rmed, double sigma, double rclamp_lo, double rclamp_hi, double base,
+                  uint64_t seed, double sink, double* q, double* r, double* height_out) {
+  Rng rng(seed);
+  std::vector<double> radii(n);
+  double vol = 0.0, rmax = 0.0, dsum = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    double x = rmed * std::exp(sigma * rng.normal());
+    x = std::min(rclamp_hi, std::max(rclamp_lo, x));
+    radii[i] = x;
+    rmax = std::max(rmax, x);
+    dsum += 2 * x;
+    vol += 4.0 / 3.0 * 3.141592653589793 * x * x * x;
+  }
+  if (n == 0) return 0;
+  if (!(base > 4 * rmax)) return -1;
+  // generous height bound for the grid: a loose pile (volume fraction 0.3) plus margin
+  const double hmax = vol / (0.3 * base * base) + 4 * rmax;
+  const double lo[3] = {-base / 2, 0.0, -base / 2}, hi[3] = {base / 2, hmax, base / 2};
+  Grid<3> grid(dsum / n, lo, hi);  // cells of the mean diameter, one sphere each on average
+  const int ncx = grid.dims[0];
+  std::vector<std::vector<uint32_t>> cols((size_t)grid.dims[0] * grid.dims[2]);
+  std::vector<double> colmax((size_t)grid.dims[0] * grid.dims[2], -1.0);
+  double top = 0.0;
+  std::vector<uint32_t> cand;
+  // spheres whose inflated surface may be reached from within `margin` of g
+  auto gather = [&](const double* g, double ri, double margin) {
+    cand.clear();
+    const double reach = ri + rmax + margin;
+    int c0[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+      c0[a] = grid.coord(g[a] - reach, a);
+      c1[a] = grid.coord(g[a] + reach, a);
+    }
+    int c[3];
+    for (c[2] = c0[2]; c[2] <= c1[2]; ++c[2])
+      for (c[1] = c0[1]; c[1] <= c1[1]; ++c[1])
+        for (c[0] = c0[0]; c[0] <= c1[0]; ++c[0])
+          for (uint32_t j : grid.cells[grid.index(c)]) {
+            const double dx = g[0] - q[3 * j], dy = g[1] - q[3 * j + 1], dz = g[2] - q[3 * j + 2];
+            const double R = ri + r[j] + margin;
+            if (dx * dx + dy * dy + dz * dz < R * R) cand.push_back(j);
+          }
+  };
+  for (uint32_t i = 0; i < n; ++i) {
+    const double ri = radii[i];
+    double p[3];
+    p[0] = -base / 2 + ri + (base - 2 * ri) * rng.uniform();
+    p[2] = -base / 2 + ri + (base - 2 * ri) * rng.uniform();
+    // vertical drop: the highest first contact along the line, or the floor
+    double y = ri;
+    {
+      const int cx0 = grid.coord(p[0] - ri - rmax, 0), cx1 = grid.coord(p[0] + ri + rmax, 0);
+      const int cz0 = grid.coord(p[2] - ri - rmax, 2), cz1 = grid.coord(p[2] + ri + rmax, 2);
+      for (int cz = cz0; cz <= cz1; ++cz)
+        for (int cx = cx0; cx <= cx1; ++cx) {
+          const size_t col = (size_t)cz * ncx + cx;
+          if (colmax[col] + ri + rmax <= y) continue;
+          for (uint32_t j : cols[col]) {
+            const double dx = p[0] - q[3 * j], dz = p[2] - q[3 * j + 2], R = ri + r[j];
+            const double h2 = dx * dx + dz * dz;
+            if (h2 < R * R) y = std::max(y, q[3 * j + 1] + std::sqrt(R * R - h2));
+          }
+        }
+    }
+    p[1] = y;
+    // roll downhill: trial moves of `step` straight down, projected out of the
+    // candidates; candidates are gathered with margin ri around g and refreshed when
+    // the sphere has moved ri/2 from g (a trial point then stays within ri of g)
+    double g[3] = {p[0], p[1], p[2]};
+    gather(g, ri, ri);
+    double step = 0.5 * ri;
+    // at most 400 trial moves: a sphere creeping along a shallow groove stops early,
+    // supported but short of its pocket (the first simulated steps finish the job)
+    for (int trial = 0; trial < 400 && step > 1e-6 * ri; ++trial) {
+      const double mx = p[0] - g[0], my = p[1] - g[1], mz = p[2] - g[2];
+      if (mx * mx + my * my + mz * mz > 0.25 * ri * ri) {
+        g[0] = p[0];
+        g[1] = p[1];
+        g[2] = p[2];
+        gather(g, ri, ri);
+      }
+      double t[3] = {p[0], p[1] - step, p[2]};
+      bool ok = false;
+      for (int pass = 0; pass < 16; ++pass) {
+        double worst = 0.0;
+        t[1] = std::max(t[1], ri);
+        for (int a = 0; a < 3; a += 2) t[a] = std::min(base / 2 - ri, std::max(-base / 2 + ri, t[a]));
+        for (uint32_t j : cand) {
+          const double dx = t[0] - q[3 * j], dy = t[1] - q[3 * j + 1], dz = t[2] - q[3 * j + 2];
+          const double R = ri + r[j];
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 < R * R) {
+            const double d = std::sqrt(d2);
+            worst = std::max(worst, R - d);
+            if (d > 0) {
+              const double sc = R / d;
+              t[0] = q[3 * j] + dx * sc;
+              t[1] = q[3 * j + 1] + dy * sc;
+              t[2] = q[3 * j + 2] + dz * sc;
+            } else {
+              t[1] = q[3 * j + 1] + R;
+            }
+          }
+        }
+        if (worst <= 1e-12 * ri && t[1] >= ri && std::fabs(t[0]) <= base / 2 - ri && std::fabs(t[2]) <= base / 2 - ri) {
+          ok = true;
+          break;
+        }
+      }
+      const double dmx = t[0] - g[0], dmy = t[1] - g[1], dmz = t[2] - g[2];
+      // keep moves that descend by a fair part of the step; shallow ones halve it (a
+      // sphere sliding down a nearly level groove would otherwise creep at full step)
+      if (ok && t[1] < p[1] - 0.05 * step && dmx * dmx + dmy * dmy + dmz * dmz < ri * ri) {
+        p[0] = t[0];
+        p[1] = t[1];
+        p[2] = t[2];
+        step = std::min(0.5 * ri, 1.5 * step);
+      } else {
+        step *= 0.5;
+      }
+    }
+    p[1] -= sink;
+    q[3 * i] = p[0];
+    q[3 * i + 1] = p[1];
+    q[3 * i + 2] = p[2];
+    r[i] = ri;
+    if (p[1] + ri > hmax - 2 * rmax) return -(int)i - 2;  // grid too short
+    int c[3];
+    for (int a = 0; a < 3; ++a) c[a] = grid.coord(p[a], a);
+    grid.cells[grid.index(c)].push_back(i);
+    const size_t col = (size_t)c[2] * ncx + c[0];
+    cols[col].push_back(i);
+    colmax[col] = std::max(colmax[col], p[1]);
+    top = std::max(top, p[1] + ri);
+  }
+  if (height_out) *height_out = top;
+  return 0;
+}
+
 }  // extern "C"

@@ -16,6 +16,16 @@
 
 using namespace scx;
 
+// Phase boundary for scx_get_timing: an event on the context stream.
+void scx::timing_mark(scx_ctx* c, const char* name) {
+  if (!c->timing) return;
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  hipEventRecord(ev, c->stream);
+  c->events.push_back(ev);
+  c->event_names.push_back(name);
+}
+
 namespace {
 
 const char* kVersion = "scx 0.1 (gfx950)";
@@ -63,6 +73,8 @@ uint32_t next_pow2(uint64_t v) {
 int alloc_bodies(scx_ctx* c, uint32_t N) {
   const int D = c->dim;
   DeviceBuffers& d = c->d;
+  // grid entries carry a 29-bit body position
+  if (N >= (1u << 29)) return fail(c, SCX_E_CAPACITY, "at most %u bodies", (1u << 29) - 1);
   HIPC(dalloc(&d.q, (size_t)D * N));
   HIPC(dalloc(&d.v, (size_t)D * N));
   HIPC(dalloc(&d.w, 3 * (size_t)N));
@@ -85,12 +97,12 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
   HIPC(dalloc(&d.pcount, (size_t)N + 1));
   HIPC(dalloc(&d.pmask, std::max<size_t>(N, c->cap_contacts)));
   HIPC(dalloc(&d.deg, 2 * ((size_t)N + 1)));
-  HIPC(dalloc(&d.sid, N));
-  HIPC(dalloc(&d.smin, (size_t)D * N));
-  HIPC(dalloc(&d.smax, (size_t)D * N));
-  HIPC(dalloc(&d.sq0, (size_t)D * N));
-  HIPC(dalloc(&d.sq1, (size_t)D * N));
-  HIPC(dalloc(&d.sr, N));
+  HIPC(dalloc(&d.order, N));
+  HIPC(dalloc(&d.order_next, N));
+  HIPC(dalloc(&d.tbox, 2 * (size_t)N));
+  HIPC(dalloc(&d.tnar, 2 * (size_t)N));
+  HIPC(dalloc(&d.tr, N));
+  c->order_valid = false;
   // dense grid budget: cells of the reference's size (mean box extent) number about N
   // for a packed scene; the grid is coarsened on the device beyond this budget
   c->cap_cells = 2 * N + 1024;
@@ -184,15 +196,17 @@ void free_all(scx_ctx* c) {
   float* flt[] = {d.bv32, d.bw32};
   for (float* p : flt)
     if (p) hipFree(p);
-  uint32_t* u32[] = {d.cells, d.ent, d.sid, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
+  uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
     if (p) hipFree(p);
   if (d.cplane) hipFree(d.cplane);
   if (d.cstate) hipFree(d.cstate);
-  double* sorted[] = {d.smin, d.smax, d.sq0, d.sq1, d.sr};
-  for (double* p : sorted)
-    if (p) hipFree(p);
+  dfree(&d.order);
+  dfree(&d.order_next);
+  dfree(&d.tbox);
+  dfree(&d.tnar);
+  dfree(&d.tr);
   if (d.pmask) hipFree(d.pmask);
   if (d.prev_key) hipFree(d.prev_key);
   if (d.cur_key) hipFree(d.cur_key);
@@ -204,15 +218,6 @@ void free_all(scx_ctx* c) {
   free_soa(d.c64);
   free_soa(d.c32);
   d = DeviceBuffers{};
-}
-
-void timing_mark(scx_ctx* c, const char* name) {
-  if (!c->timing) return;
-  hipEvent_t ev;
-  if (hipEventCreate(&ev) != hipSuccess) return;
-  hipEventRecord(ev, c->stream);
-  c->events.push_back(ev);
-  c->event_names.push_back(name);
 }
 
 void timing_collect(scx_ctx* c) {
@@ -245,6 +250,10 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   const int D = c->dim;
   if (c->events.empty()) timing_mark(c, "start");
   HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+  if (!c->order_valid) {
+    HIPC(launch_order_reset(c, n));
+    c->order_valid = true;
+  }
   // start-of-step state q0, v0, omega0 (the resident state becomes the step input)
   std::swap(d.q, d.q0);
   std::swap(d.v, d.v0);
@@ -253,14 +262,18 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   HIPC(launch_predict_aabb(c, d.q0, d.v0, d.m, p->dt, p->g, p->integrator, d.q1p));
   timing_mark(c, "predict");
   // broad phase binning
-  HIPC(launch_grid_build(c, n, false, true));
+  HIPC(launch_grid_build(c, n, false));
   timing_mark(c, "grid");
   // body-body active set (Ball2DSim.cpp:553-608 / RigidBody3DSim.cpp:1072-1260)
   const int filter = (D == 2) ? 1 : 2;
   HIPC(launch_pairs(c, n, 0, filter, d.ca, d.cb, c->cap_contacts));
+  timing_mark(c, "pairs_count");
   HIPC(launch_exclusive_scan(d.pcount, d.pcount, n, d.scan_tmp, s));
+  timing_mark(c, "pairs_scan");
   HIPC(launch_pairs(c, n, 1, filter, d.ca, d.cb, c->cap_contacts));
-  timing_mark(c, "pairs");
+  timing_mark(c, "pairs_write");
+  // this binning's home-cell order is the next step's sweep order
+  std::swap(d.order, d.order_next);
   // plane contacts, plane-major (Ball2DSim.cpp:747-764 / RigidBody3DSim.cpp:1414-1502)
   uint32_t npl = 0;
   if (c->nplanes > 0) {
@@ -311,7 +324,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     timing_mark(c, "gather");
   }
   HIPC(launch_solve(c, C, max_colour, p));
-  timing_mark(c, "solve");
+  timing_mark(c, "gs_fp64");
   HIPC(launch_finish(c, C, p));
   timing_mark(c, "finish");
   c->last_ncontacts = C;
@@ -332,6 +345,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
       st->sweeps_fp32 = d.ctr_host->sweeps_fp32;
       st->error = (C == 0) ? 0.0 : d.ctr_host->error;
       st->succeeded = (C == 0) ? 1 : (d.ctr_host->error < p->tol ? 1 : 0);
+      st->ncells = d.ctr_host->ncells;
     }
   }
   timing_collect(c);
@@ -482,6 +496,7 @@ int scx_set_bodies(scx_ctx* c, uint32_t n, const double* q, const double* v, con
   HIPC(hipStreamSynchronize(s));
   c->n = n;
   c->has_bodies = true;
+  c->order_valid = false;
   c->last_ncontacts = 0;
   return SCX_OK;
 }
@@ -773,7 +788,8 @@ int scx_potential_overlaps(scx_ctx* c, uint32_t n, const double* aabb_min, const
   HIPC(hipMemcpyAsync(d.aabb_max, aabb_max, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
   for (int attempt = 0; attempt < 8; ++attempt) {
     HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
-    HIPC(launch_grid_build(c, n, true, false));
+    c->order_valid = false;  // the sweep order now belongs to these boxes
+    HIPC(launch_grid_build(c, n, true));
     HIPC(launch_pairs(c, n, 0, 0, nullptr, nullptr, 0));
     HIPC(launch_exclusive_scan(d.pcount, d.pcount, n, d.scan_tmp, s));
     HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));

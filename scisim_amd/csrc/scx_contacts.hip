@@ -329,7 +329,8 @@ hipError_t launch_colouring(scx_ctx* c, uint32_t C) {
   if (e != hipSuccess) return e;
   if (nblk) k_adj_fill<<<nblk, kBlock, 0, s>>>(C, c->d.ca, c->d.cb, deg, off, c->d.adj);
   if (C == 0) return hipGetLastError();
-  const int blocks = (int)std::min<uint32_t>((uint32_t)c->colour_blocks, nblk);
+  // small contact sets: one workgroup, whose rounds synchronise with workgroup barriers
+  const int blocks = (C <= 16 * kBlock) ? 1 : (int)std::min<uint32_t>((uint32_t)c->colour_blocks, nblk);
   uint32_t Cv = C;
   const uint32_t* pa = c->d.ca;
   const uint32_t* pb = c->d.cb;

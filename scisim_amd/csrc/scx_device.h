@@ -76,6 +76,12 @@ constexpr uint32_t kBarrierMaxGroups = 64;
 constexpr uint32_t kBarrierWords = 64 + 32 * kBarrierMaxGroups;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
+  // one workgroup: its waves share the CU's L1, a workgroup barrier orders their
+  // global-memory accesses (no device-scope fence or atomics needed)
+  if (gridDim.x == 1) {
+    __syncthreads();
+    return;
+  }
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -125,6 +131,15 @@ __device__ __forceinline__ uint32_t cell_of(double x, double mn, double h, uint3
   const double t = (x - mn) / h;
   if (!(t > 0.0)) return 0u;
   if (!(t < (double)(dim - 1))) return dim - 1;
+  return (uint32_t)t;
+}
+
+// FP32 cell index, clamped to [0, dim - 1]. Non-decreasing in x (every step rounds
+// monotonically), which is all the binning needs; NaN maps to cell 0.
+__device__ __forceinline__ uint32_t cell_of_f(float x, float mn, float inv_h, uint32_t dim) {
+  const float t = (x - mn) * inv_h;
+  if (!(t > 0.0f)) return 0u;
+  if (!(t < (float)(dim - 1))) return dim - 1;
   return (uint32_t)t;
 }
 

@@ -26,6 +26,8 @@ struct GridParams {
   double h;
   uint32_t dims[3];      // cells per axis (dense row-major grid)
   uint32_t ncells;
+  float min_f[3];        // FP32 copies used to bin the FP32 boxes
+  float inv_h_f;
 };
 
 // Device-side counters of one step, read back by the host at the sync points.
@@ -37,6 +39,8 @@ struct StepCounters {
   uint32_t remaining;            // uncoloured contacts
   uint32_t sweeps;               // total sweeps
   uint32_t sweeps_fp32;
+  uint32_t ncells;               // cells of this step's grid
+  uint32_t pad;
   double error;                  // best residual of the last phase
   double error_fp32;
 };
@@ -87,13 +91,15 @@ struct DeviceBuffers {
   double* red = nullptr;       // reduction partials
   GridParams* grid = nullptr;
   uint32_t* cells = nullptr;   // 4 x (cap_cells + 1): entry counts, entry starts, home counts, home starts
-  uint32_t* ent = nullptr;     // cell entries: sorted body index | first-cell flags
-  uint32_t* sid = nullptr;     // home-cell order -> body id
-  double* smin = nullptr;      // boxes in home-cell order (dim*N)
-  double* smax = nullptr;
-  double* sq0 = nullptr;       // narrow-phase data in home-cell order
-  double* sq1 = nullptr;
-  double* sr = nullptr;
+  uint32_t* ent = nullptr;     // cell entries: sweep position t | first-cell flags
+  // Sweep order: order[t] = body id, the home-cell order of the previous binning (any
+  // permutation is correct; a spatially sorted one keeps the atomics and the neighbour
+  // gathers local). order_next receives this step's home-cell order.
+  uint32_t* order = nullptr;
+  uint32_t* order_next = nullptr;
+  float4* tbox = nullptr;      // conservative FP32 boxes in sweep order (2D: 1, 3D: 2 per body)
+  double4* tnar = nullptr;     // exact FP64 narrow-phase data in sweep order (2 per body, see k_pairs)
+  double* tr = nullptr;        // radii in sweep order (2D)
   uint32_t* pcount = nullptr;  // N + 1 per-owner counts / offsets
   uint32_t* scan_tmp = nullptr; // scratch of the device-wide scans
   uint32_t* pa = nullptr;      // pair lists of scx_potential_overlaps (capacity cap_pairs)
@@ -148,6 +154,7 @@ struct scx_ctx {
   uint32_t cap_perm = 0;  // perm buffer length (plane compaction and colour order)
   uint32_t n = 0, nplanes = 0;
   bool has_bodies = false;
+  bool order_valid = false;  // d.order holds a permutation of the current n bodies
   uint32_t last_ncontacts = 0, last_npairs = 0, last_nplane = 0, last_ncolours = 0;
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -163,12 +170,17 @@ struct scx_ctx {
 namespace scx {
 hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t s);
 size_t scan_tmp_words(uint32_t n);
+// Records a phase boundary (HIP event on the context stream) when timing is enabled.
+void timing_mark(scx_ctx* c, const char* name);
 
 hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, const double* m, double dt,
                                const double* g, int integrator, double* q1p);
 hipError_t launch_flow(scx_ctx* c, uint32_t n, int integrator, const double* m, const double* g, double dt,
                        const double* q0, const double* v0, double* q1, double* v1);
-hipError_t launch_grid_build(scx_ctx* c, uint32_t n, bool given_boxes, bool narrow_data);
+// given_boxes: bin the uploaded boxes d.aabb_min/max (scx_potential_overlaps);
+// otherwise the predictor has written the sweep-order records of this step.
+hipError_t launch_grid_build(scx_ctx* c, uint32_t n, bool given_boxes);
+hipError_t launch_order_reset(scx_ctx* c, uint32_t n);
 hipError_t launch_pairs(scx_ctx* c, uint32_t n, int mode, int filter, uint32_t* out_a, uint32_t* out_b, uint32_t cap);
 hipError_t launch_plane_masks(scx_ctx* c, uint32_t n);
 hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint32_t n, uint32_t nbuckets,

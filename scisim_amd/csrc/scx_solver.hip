@@ -468,9 +468,15 @@ __global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, co
 // Persistent launch sized to the largest colour: one contact per thread per colour
 // pass, so the barrier (whose cost grows with the number of workgroups) spans no
 // more workgroups than the widest pass needs.
+// Up to kGSSingleMax contacts per colour one workgroup sweeps them in a few passes:
+// its barrier is a workgroup barrier (~0.1 us) instead of a device-wide one (several us
+// across XCDs), which wins while the colours are a few thousand contacts wide.
+constexpr uint32_t kGSSingleMax = 8 * kGSBlock;
+
 template <int D, typename Real>
 static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, uint32_t max_colour) {
-  const uint32_t want = (std::max<uint32_t>(max_colour, 1u) + kGSBlock - 1) / kGSBlock;
+  const uint32_t want =
+      (max_colour <= kGSSingleMax) ? 1u : (std::max<uint32_t>(max_colour, 1u) + kGSBlock - 1) / kGSBlock;
   const int blocks = (int)std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)max_blocks, want));
   void* args[] = {&A};
   return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
@@ -495,6 +501,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
     k_body_init<D><<<nb, kBlock, 0, s>>>(c->n, c->d.v0, c->d.w0, c->d.m, c->d.inertia, c->d.bv64, c->d.bw64,
                                           c->d.bv32, c->d.bw32, fp32 ? 1 : 0);
   hipError_t e = hipGetLastError();
+  timing_mark(c, "gs_init");
   if (e != hipSuccess || C == 0) return e;
   if (fp32) {
     GSArgs<float> A{c->d.c32, c->d.bv32, c->d.bw32, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
@@ -502,10 +509,12 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
     e = coop_gs<D, float>(c, A, c->gs_blocks, max_colour);
     if (e != hipSuccess) return e;
+    timing_mark(c, "gs_fp32");
     const uint32_t nr = (3 * C + kBlock - 1) / kBlock;
     k_promote_r<<<nr, kBlock, 0, s>>>(C, c->d.c32.r, c->d.c64.r);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    timing_mark(c, "gs_promote");
   }
   GSArgs<double> B{c->d.c64, c->d.bv64, c->d.bw64, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,

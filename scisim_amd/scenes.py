@@ -23,6 +23,8 @@ def lib():
                                 ctypes.c_uint64, _pd, _pd, _pd]
         L.scn_rsa3d.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64, _pd, _pd, _pd]
+        L.scn_deposit3d.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, _pd, _pd, _pd]
         _lib = L
     return _lib
 
@@ -87,6 +89,28 @@ def spheres_rsa(n=262144, rmed=0.05, sigma=0.25, base=None, phi=0.3, seed=1, den
                          max_attempts, _p(q), _p(r), ctypes.byref(height))
     if rc != 0:
         raise RuntimeError("RSA packing failed at sphere %d" % (-rc - 1))
+    m = density * (4.0 * np.pi * r * r * r / 3.0)
+    px, pn = box_planes_3d(base / 2)
+    return {"dim": 3, "q": q, "v": np.zeros(3 * n), "r": r, "m": m, "inertia": 2.0 * m * r * r / 5.0, "px": px,
+            "pn": pn, "base": base, "height": height.value}
+
+
+def spheres_pile(n=262144, rmed=0.05, sigma=0.25, base=None, seed=1, density=1.0, sink=1e-9):
+    """A poured pile at rest: log-normal spheres dropped one by one onto the floor of a
+    box (floor + 4 walls) and rolled downhill into a pocket of their supports (about six
+    contacts per sphere). Each sphere overlaps its supports by `sink` so that its contacts
+    are active from the first step. v = 0."""
+    if base is None:
+        # base so that the pile (volume fraction ~0.58) is roughly as tall as wide
+        vol = n * 4.0 / 3.0 * np.pi * rmed ** 3 * np.exp(4.5 * sigma * sigma)
+        base = (vol / 0.58) ** (1.0 / 3.0)
+    q = np.empty(3 * n)
+    r = np.empty(n)
+    height = ctypes.c_double(0.0)
+    rc = lib().scn_deposit3d(n, rmed, sigma, rmed * np.exp(-5 * sigma), rmed * np.exp(5 * sigma), base, seed, sink,
+                             _p(q), _p(r), ctypes.byref(height))
+    if rc != 0:
+        raise RuntimeError("deposition failed (%d)" % rc)
     m = density * (4.0 * np.pi * r * r * r / 3.0)
     px, pn = box_planes_3d(base / 2)
     return {"dim": 3, "q": q, "v": np.zeros(3 * n), "r": r, "m": m, "inertia": 2.0 * m * r * r / 5.0, "px": px,

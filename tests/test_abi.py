@@ -63,3 +63,29 @@ def test_scene_generators_are_deterministic_and_non_overlapping():
     np.fill_diagonal(d, np.inf)
     assert np.all(d >= s["r"][:, None] + s["r"][None, :])
     assert np.all(q[:, 1] - s["r"] >= 0.0)
+
+
+def test_poured_pile_rests_on_its_supports():
+    from scisim_amd import scenes
+    sink = 1e-9
+    a = scenes.spheres_pile(n=1500, seed=4, sink=sink)
+    b = scenes.spheres_pile(n=1500, seed=4, sink=sink)
+    assert np.array_equal(a["q"], b["q"]) and np.array_equal(a["r"], b["r"])
+    q, r = a["q"].reshape(-1, 3), a["r"]
+    d = np.sqrt(((q[:, None, :] - q[None, :, :]) ** 2).sum(-1))
+    np.fill_diagonal(d, np.inf)
+    gap = d - (r[:, None] + r[None, :])
+    # overlaps only by the requested sink (plus rounding)
+    assert gap.min() >= -sink * (1 + 1e-3) - 1e-12
+    # inside the walls, above the floor (up to the sink)
+    assert np.all(np.abs(q[:, [0, 2]]) <= a["base"] / 2 - r[:, None] + 1e-12)
+    assert np.all(q[:, 1] - r >= -sink * (1 + 1e-3))
+    # spheres rest on the floor or on a sphere below their centre (the rolling of a few
+    # is cut short by its trial budget; the first simulated steps settle those)
+    touching = gap <= 1e-7
+    below = q[None, :, 1] < q[:, None, 1]
+    supported = (q[:, 1] - r <= 1e-7) | np.any(touching & below, axis=1)
+    assert supported.mean() > 0.99
+    # a pile, not a gas: about four contacts per sphere or more before settling
+    contacts = (touching.sum() + (q[:, 1] - r <= 1e-7).sum()) / len(r)
+    assert contacts > 3.5
