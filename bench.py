@@ -6,8 +6,8 @@ Workloads (synthetic, seeded):
         CoR = 0, dt = 0.005, symplectic Euler, FP32 sweeps + FP64 residual/polish,
         residual tolerance 1e-6.
   cfg2: rigidbody3d, 262,144 spheres, log-normal radii (median 0.05, sigma 0.25),
-        poured into a box with 5 planes (random sequential packing at 30% then
-        untimed settling steps), mu = 0.4, CoR = 0, Verlet, dt = 0.005.
+        poured into a box with 5 planes (sequential deposition into a resting pile,
+        then untimed settling steps), mu = 0.4, CoR = 0, Verlet, dt = 0.005.
 
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
 narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final
@@ -39,12 +39,12 @@ def make_scene(cfg, seed):
                 "packing": "RSA phi=0.55", "planes": 4, "mu": 0.3, "CoR": 0.0, "dt": 0.005,
                 "integrator": "symplectic_euler", "tol": 1e-6}
     else:
-        s = scenes.spheres_rsa(n=262144, rmed=0.05, sigma=0.25, phi=0.3, seed=seed)
+        s = scenes.spheres_pile(n=262144, rmed=0.05, sigma=0.25, seed=seed)
         prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1,
                    integrator=1)
-        desc = {"workload": "spheres3d_pour_262k", "bodies": 262144, "dim": 3, "radii": "lognormal(0.05,0.25)",
-                "packing": "RSA phi=0.3 then settled", "planes": 5, "mu": 0.4, "CoR": 0.0, "dt": 0.005,
-                "integrator": "verlet", "tol": 1e-6}
+        desc = {"workload": "spheres3d_pile_262k", "bodies": 262144, "dim": 3, "radii": "lognormal(0.05,0.25)",
+                "packing": "poured pile (sequential deposition), settled", "planes": 5, "mu": 0.4, "CoR": 0.0,
+                "dt": 0.005, "integrator": "verlet", "tol": 1e-6}
     return s, prm, desc
 
 
@@ -69,7 +69,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--settle", type=int, default=200, help="untimed settling steps (cfg2)")
+    ap.add_argument("--settle", type=int, default=20, help="untimed settling steps (cfg2)")
+    ap.add_argument("--warm", default="none", choices=["none", "cache", "normal", "transport"],
+                    help="impulse cache warm start (ImpulsesToCache)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--max-iters", type=int, default=None, help="override the sweep limit (profiling)")
     ap.add_argument("--eval-every", type=int, default=None, help="override the residual period (profiling)")
@@ -99,7 +101,7 @@ def main():
     precision = scx.FP32_POLISH if args.precision == "fp32" else scx.FP64
     it = 1
     if args.config == "cfg2" and args.settle > 0:
-        sp = scx.step_params(precision=precision, **dict(prm, max_iters=200))
+        sp = scx.step_params(precision=precision, **dict(prm, max_iters=500))
         t0 = time.perf_counter()
         ctx.run(sp, args.settle)
         log("settled %d steps in %.1fs" % (args.settle, time.perf_counter() - t0))
@@ -108,6 +110,9 @@ def main():
         prm["max_iters"] = args.max_iters
     if args.eval_every is not None:
         prm["eval_every"] = args.eval_every
+    prm["warm_start"] = {"none": scx.WARM_NONE, "cache": scx.WARM_CACHE, "normal": scx.WARM_NORMAL,
+                         "transport": scx.WARM_TRANSPORT}[args.warm]
+    desc["warm_start"] = args.warm
     params = scx.step_params(precision=precision, iteration=it, **prm)
     for _ in range(args.warmup):
         ctx.step(params)

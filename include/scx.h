@@ -49,8 +49,14 @@ extern "C" {
 #define SCX_PRECISION_FP64 0       /* every sweep in FP64 */
 #define SCX_PRECISION_FP32_POLISH 1 /* FP32 sweeps, then FP64 polishing sweeps to tolerance */
 
+/* Impulse cache between steps (ImpulsesToCache, GeometricImpactFrictionMap.cpp:174-264):
+ * the previous step's impulses of the same constraint, (i, j) or (plane, i), warm start
+ * the solve; constraints new this step start from zero. */
 #define SCX_WARM_NONE 0      /* ImpulsesToCache::NONE */
-#define SCX_WARM_CACHE 1     /* previous-step impulses carried by contact key (ImpulsesToCache::NORMAL_AND_FRICTION) */
+#define SCX_WARM_CACHE 1     /* ImpulsesToCache::NORMAL_AND_FRICTION: alpha and beta (2D only, as the reference) */
+#define SCX_WARM_NORMAL 2    /* ImpulsesToCache::NORMAL: alpha only (2D and 3D) */
+#define SCX_WARM_TRANSPORT 3 /* world impulse rotated with the contact normal, then projected on the
+                                new basis (the 3D cache of SymplecticEulerImpactFrictionMap.cpp:87-127) */
 
 typedef struct scx_ctx scx_ctx;
 

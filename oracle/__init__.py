@@ -55,6 +55,8 @@ def lib():
         L.orc_fb_err.restype = _d
         L.orc_fb_err.argtypes = [ctypes.c_int, _d, _pd, _pd]
         L.orc_local_solve.argtypes = [ctypes.c_int, _d, _d, _d, _pd, _pd]
+        L.orc_transport.argtypes = [ctypes.c_int, _pd, _pd, _pd, _pd]
+        L.orc_transport.restype = None
         L.orc_umap_flow.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _pd, _pd, _d, _pd, _pd, _pd, _pd]
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int]
@@ -131,6 +133,16 @@ def ccd_happens(c):
 
 def fb_err(dim, mu, r, u):
     return lib().orc_fb_err(dim, mu, _p(_f64(r)), _p(_f64(u)))
+
+
+def transport(n_old, r_old, basis):
+    """Cached world impulse r_old (with its normal n_old) rotated onto the normal of
+    `basis` (D x D, columns n s t) and expressed in that basis."""
+    B = _f64(basis)
+    dim = B.shape[0]
+    out = np.zeros(dim)
+    lib().orc_transport(dim, _p(_f64(n_old)), _p(_f64(r_old)), _p(np.ascontiguousarray(B.T).ravel()), _p(out))
+    return out
 
 
 def local_solve(dim, mu, wn, wt, bt):

@@ -26,6 +26,7 @@
 //     at that level; results are compared at the residual tolerance instead.
 
 #include <algorithm>
+#include <array>
 #include <cassert>
 #include <cmath>
 #include <cstdint>
@@ -232,7 +233,8 @@ struct Params {
   uint32_t eval_every;
   int integrator;  // 0 symplectic Euler, 1 Verlet (3D: SplitHam translational part)
   int order;       // 0 natural active-set order, 1 colour order
-  int warm_start;  // 0 none (ImpulsesToCache::NONE), 1 normal+friction cache (2D)
+  int warm_start;  // ImpulsesToCache: 0 NONE, 1 NORMAL_AND_FRICTION (2D only), 2 NORMAL,
+                   // 3 world impulse transported with the normal (SE-map cache)
 };
 
 // Fischer-Burmeister residual of one contact with the De Saxce change of variable
@@ -636,6 +638,19 @@ void orc_ccd_coeffs(const double* q0a, const double* q1a, double ra, const doubl
 
 int orc_ccd_happens(const double* c, double* t) { return ccd_happens(c, t) ? 1 : 0; }
 
+void transport_impulse(int D, const double* n_old, const double* r_old, const double* B, double* f);
+
+// Transported cached impulse in the local frame of basis B (column-major [n s t]).
+void orc_transport(int dim, const double* n_old, const double* r_old, const double* B, double* rloc) {
+  double f[3];
+  transport_impulse(dim, n_old, r_old, B, f);
+  for (int k = 0; k < dim; ++k) {
+    double acc = B[k * dim] * f[0];
+    for (int a = 1; a < dim; ++a) acc = acc + B[k * dim + a] * f[a];
+    rloc[k] = acc;
+  }
+}
+
 double orc_fb_err(int dim, double mu, const double* r, const double* u) {
   return dim == 2 ? fb_err<2>(mu, r, u) : fb_err<3>(mu, r, u);
 }
@@ -785,13 +800,59 @@ static void active_set(int dim, int n, const double* q0, const double* q1, const
   }
 }
 
+// Previous world impulse r_old rotated by the minimal rotation taking the old normal
+// n_old to the current normal B[0..D) (Eigen's Quaternion::FromTwoVectors and
+// quaternion-vector product, as SymplecticEulerImpactFrictionMap.cpp:96-115 uses them);
+// an all-zero cache entry gives zero. 2D vectors are rotated in the xy-plane.
+void transport_impulse(int D, const double* n_old, const double* r_old, const double* B, double* f) {
+  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+  bool zero = true;
+  for (int k = 0; k < D; ++k) {
+    a[k] = n_old[k];
+    b[k] = B[k];
+    r[k] = r_old[k];
+    zero = zero && n_old[k] == 0.0 && r_old[k] == 0.0;
+  }
+  f[0] = f[1] = f[2] = 0.0;
+  if (zero) return;
+  const double na = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  const double nb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  for (int k = 0; k < 3; ++k) {
+    a[k] = a[k] / na;
+    b[k] = b[k] / nb;
+  }
+  const double c = b[0] * a[0] + b[1] * a[1] + b[2] * a[2];
+  double v[3], w;
+  if (c < -1.0 + 1e-12) {
+    // opposite normals: half turn about an axis orthogonal to the old normal
+    double ax[3];
+    orthogonal_vector(a, ax);
+    const double w2 = (1.0 + std::max(c, -1.0)) * 0.5;
+    w = std::sqrt(w2);
+    const double sv = std::sqrt(1.0 - w2);
+    for (int k = 0; k < 3; ++k) v[k] = ax[k] * sv;
+  } else {
+    const double ax[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double s = std::sqrt((1.0 + c) * 2.0);
+    const double invs = 1.0 / s;
+    for (int k = 0; k < 3; ++k) v[k] = ax[k] * invs;
+    w = s * 0.5;
+  }
+  double uv[3] = {v[1] * r[2] - v[2] * r[1], v[2] * r[0] - v[0] * r[2], v[0] * r[1] - v[1] * r[0]};
+  for (int k = 0; k < 3; ++k) uv[k] = uv[k] + uv[k];
+  const double vu[3] = {v[1] * uv[2] - v[2] * uv[1], v[2] * uv[0] - v[0] * uv[2], v[0] * uv[1] - v[1] * uv[0]};
+  for (int k = 0; k < 3; ++k) f[k] = (r[k] + w * uv[k]) + vu[k];
+}
+
 // Persistent oracle context: holds the 2D warm-start cache between steps
 // (ball2d/ConstraintCache.cpp:20-123; GeometricImpactFrictionMap.cpp:174-264) and the
 // last step's contacts and impulses for inspection.
 typedef struct orc_ctx {
   int dim;
-  std::map<std::pair<int, int>, std::pair<double, double>> cache_bb;  // (i,j) -> (alpha, beta)
-  std::map<std::pair<int, int>, std::pair<double, double>> cache_pl;  // (plane,i) -> (alpha, beta)
+  // (i,j) / (plane,i) -> (alpha, beta, 0...) for NORMAL / NORMAL_AND_FRICTION, or
+  // (old normal, world impulse) for the transported cache
+  std::map<std::pair<int, int>, std::array<double, 6>> cache_bb;
+  std::map<std::pair<int, int>, std::array<double, 6>> cache_pl;
   std::vector<Contact> cs;
   std::vector<int> colour;
   std::vector<double> alpha, beta, rworld;
@@ -814,6 +875,8 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
              orc_stats* st) {
   const int D = ctx->dim;
   std::memset(st, 0, sizeof(*st));
+  // "Caching in 3 space not yet supported." (GeometricImpactFrictionMap.cpp:258-262)
+  if (P->warm_start == 1 && D == 3) return -3;
   std::vector<double> q0(q, q + (size_t)D * n), v0(v, v + (size_t)D * n);
   std::vector<double> w0;
   if (D == 3) w0.assign(w, w + (size_t)3 * n);
@@ -845,20 +908,35 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
     return 0;
   }
   std::vector<double> mu(C, P->mu);
-  // warm start (:174-220)
+  // warm start (initializeImpulses, :174-220): the cached (alpha, beta) of the same
+  // constraint, zero on a miss (ball2d/ConstraintCache.cpp:71-123); NORMAL restores alpha
+  // only. NORMAL_AND_FRICTION exists in 2D only (the reference exits in 3D, :212-216).
+  // The transported cache (warm_start 3) keeps the old normal and world impulse and
+  // rotates the impulse with the normal (SymplecticEulerImpactFrictionMap.cpp:87-127).
   std::vector<double> rloc((size_t)D * C, 0.0);
-  if (P->warm_start && D == 2) {
+  const bool caching = (P->warm_start == 1 && D == 2) || P->warm_start == 2 || P->warm_start == 3;
+  if (caching) {
     for (int ci = 0; ci < C; ++ci) {
       const auto key = cs[ci].b >= 0 ? std::make_pair(cs[ci].a, cs[ci].b) : std::make_pair(cs[ci].plane, cs[ci].a);
       const auto& cache = cs[ci].b >= 0 ? ctx->cache_bb : ctx->cache_pl;
       auto it = cache.find(key);
-      if (it != cache.end()) {
-        // r = alpha n + beta t (Sobogus.cpp:301) then local coords E^T r
-        const double* B = cs[ci].basis;
-        double rw[2] = {it->second.first * B[0] + it->second.second * B[2],
-                        it->second.first * B[1] + it->second.second * B[3]};
-        rloc[2 * ci] = B[0] * rw[0] + B[1] * rw[1];
-        rloc[2 * ci + 1] = B[2] * rw[0] + B[3] * rw[1];
+      if (it == cache.end()) continue;
+      const double* B = cs[ci].basis;
+      const double* cv = it->second.data();
+      double rw[3] = {0, 0, 0};
+      if (P->warm_start == 3) {
+        transport_impulse(D, cv, cv + 3, B, rw);
+      } else {
+        // r = alpha n + beta t (Sobogus.cpp:301; beta = 0 for NORMAL)
+        for (int a = 0; a < D; ++a) rw[a] = cv[0] * B[a];
+        if (P->warm_start == 1)
+          for (int a = 0; a < D; ++a) rw[a] = rw[a] + cv[1] * B[D + a];
+      }
+      // local coordinates E^T r
+      for (int k = 0; k < D; ++k) {
+        double acc = B[k * D] * rw[0];
+        for (int a = 1; a < D; ++a) acc = acc + B[k * D + a] * rw[a];
+        rloc[(size_t)D * ci + k] = acc;
       }
     }
   }
@@ -911,10 +989,23 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
       ctx->beta[(size_t)(D - 1) * ci + (k - 1)] = (P->mu == 0.0) ? 0.0 : be;
     }
   }
-  // cache (:222-264)
-  if (P->warm_start && D == 2)
+  // cache (cacheImpulses, :222-264): alpha (NORMAL), (alpha, beta) (NORMAL_AND_FRICTION),
+  // or the normal and world impulse alpha n + sum_k beta_k t_k (transported)
+  if (caching)
     for (int ci = 0; ci < C; ++ci) {
-      const auto val = std::make_pair(ctx->alpha[ci], ctx->beta[ci]);
+      std::array<double, 6> val{};
+      if (P->warm_start == 3) {
+        const double* B = cs[ci].basis;
+        for (int a = 0; a < D; ++a) {
+          val[a] = B[a];
+          double w = ctx->alpha[ci] * B[a];
+          for (int k = 1; k < D; ++k) w = w + ctx->beta[(size_t)(D - 1) * ci + k - 1] * B[k * D + a];
+          val[3 + a] = w;
+        }
+      } else {
+        val[0] = ctx->alpha[ci];
+        val[1] = P->warm_start == 1 ? ctx->beta[ci] : 0.0;
+      }
       if (cs[ci].b >= 0)
         ctx->cache_bb[std::make_pair(cs[ci].a, cs[ci].b)] = val;
       else

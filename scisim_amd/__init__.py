@@ -27,6 +27,8 @@ FP64 = 0
 FP32_POLISH = 1
 WARM_NONE = 0
 WARM_CACHE = 1
+WARM_NORMAL = 2
+WARM_TRANSPORT = 3
 
 # Every symbol include/scx.h declares.
 EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_potential_overlaps", "scx_set_bodies",

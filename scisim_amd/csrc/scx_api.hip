@@ -87,10 +87,11 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
   HIPC(dalloc(&d.q1p, (size_t)D * N));
   HIPC(dalloc(&d.v2, (size_t)D * N));
   HIPC(dalloc(&d.w2, 3 * (size_t)N));
-  HIPC(dalloc(&d.bv64, 4 * (size_t)N));
-  HIPC(dalloc(&d.bw64, 4 * (size_t)N));
-  HIPC(dalloc(&d.bv32, 4 * (size_t)N));
-  HIPC(dalloc(&d.bw32, 4 * (size_t)N));
+  HIPC(dalloc(&d.bv64, 8 * (size_t)N));
+  HIPC(dalloc(&d.bv32, 8 * (size_t)N));
+  HIPC(dalloc(&d.slot_of, N));
+  d.sid = nullptr;
+  d.slot_map = nullptr;
   HIPC(dalloc(&d.aabb_min, (size_t)D * N));
   HIPC(dalloc(&d.aabb_max, (size_t)D * N));
   HIPC(dalloc(&d.red, 9 * (size_t)((N + kBlock - 1) / kBlock + 1)));
@@ -165,6 +166,7 @@ int alloc_contacts(scx_ctx* c, uint32_t C) {
   HIPC(dalloc(&d.calpha, C));
   HIPC(dalloc(&d.cbeta, (size_t)(D - 1) * C));
   HIPC(dalloc(&d.crinit, (size_t)D * C));
+  HIPC(dalloc(&d.cur_key, C));
   HIPC(dalloc(&d.adj, 2 * (size_t)C));
   HIPC(dalloc(&d.cstate, C));
   c->cap_perm = std::max<uint32_t>(C, c->cap_bodies);
@@ -188,14 +190,13 @@ int alloc_scan_tmp(scx_ctx* c) {
 
 void free_all(scx_ctx* c) {
   DeviceBuffers& d = c->d;
-  double* dbl[] = {d.q, d.v, d.w, d.r, d.m, d.inertia, d.q0, d.v0, d.w0, d.q1p, d.v2, d.w2, d.bv64, d.bw64,
+  double* dbl[] = {d.q, d.v, d.w, d.r, d.m, d.inertia, d.q0, d.v0, d.w0, d.q1p, d.v2, d.w2, d.bv64,
                    d.px, d.pn, d.aabb_min, d.aabb_max, d.red, d.basis, d.cla, d.clb, d.cwn, d.cwt, d.cforcing,
                    d.calpha, d.cbeta, d.crinit, d.rbest, d.partial, d.prev_r};
   for (double* p : dbl)
     if (p) hipFree(p);
-  float* flt[] = {d.bv32, d.bw32};
-  for (float* p : flt)
-    if (p) hipFree(p);
+  if (d.bv32) hipFree(d.bv32);
+  dfree(&d.slot_of);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -272,8 +273,10 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   timing_mark(c, "pairs_scan");
   HIPC(launch_pairs(c, n, 1, filter, d.ca, d.cb, c->cap_contacts));
   timing_mark(c, "pairs_write");
-  // this binning's home-cell order is the next step's sweep order
+  // this binning's home-cell order is the next step's sweep order, and this step's body
+  // slots for the Gauss-Seidel sweeps
   std::swap(d.order, d.order_next);
+  HIPC(launch_slots(c, n));
   // plane contacts, plane-major (Ball2DSim.cpp:747-764 / RigidBody3DSim.cpp:1414-1502)
   uint32_t npl = 0;
   if (c->nplanes > 0) {
@@ -312,6 +315,8 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   uint32_t max_colour = 0;
   if (C > 0) {
     HIPC(launch_contact_setup(c, nbb, npl, p->CoR, p->mu));
+    // warm start from the impulse cache (GeometricImpactFrictionMap.cpp:306)
+    if (p->warm_start != SCX_WARM_NONE) HIPC(launch_warm_start(c, C, p->warm_start));
     timing_mark(c, "setup");
     HIPC(launch_colouring(c, C));
     HIPC(launch_colour_sort(c, C));
@@ -320,12 +325,26 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     HIPC(hipStreamSynchronize(s));
     for (int b = 0; b < kMaxColours; ++b) max_colour = std::max(max_colour, d.host_u32[3 + b] - d.host_u32[2 + b]);
     timing_mark(c, "colour");
-    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr, nullptr));
+    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr,
+                               p->warm_start != SCX_WARM_NONE ? d.crinit : nullptr));
     timing_mark(c, "gather");
   }
   HIPC(launch_solve(c, C, max_colour, p));
   timing_mark(c, "gs_fp64");
   HIPC(launch_finish(c, C, p));
+  // cache this step's impulses (GeometricImpactFrictionMap.cpp:419); an empty active set
+  // leaves the cache empty
+  d.prev_count = 0;
+  if (p->warm_start != SCX_WARM_NONE && C > 0) {
+    if (C > d.cap_cache) {
+      const uint32_t want = C + C / 4 + 1024;
+      HIPC(dalloc(&d.prev_key, want));
+      HIPC(dalloc(&d.prev_r, (size_t)6 * want));  // kCacheWords per entry
+      d.cap_cache = want;
+    }
+    HIPC(launch_cache_store(c, C, p->warm_start));
+    d.prev_count = C;
+  }
   timing_mark(c, "finish");
   c->last_ncontacts = C;
   c->last_npairs = nbb;
@@ -363,7 +382,11 @@ int check_params(scx_ctx* c, const scx_step_params* p) {
     return fail(c, SCX_E_ARG, "unknown integrator %d", p->integrator);
   if (p->precision != SCX_PRECISION_FP64 && p->precision != SCX_PRECISION_FP32_POLISH)
     return fail(c, SCX_E_ARG, "unknown precision mode %d", p->precision);
-  if (p->warm_start != SCX_WARM_NONE) return fail(c, SCX_E_ARG, "warm start mode %d is not available", p->warm_start);
+  if (p->warm_start < SCX_WARM_NONE || p->warm_start > SCX_WARM_TRANSPORT)
+    return fail(c, SCX_E_ARG, "unknown warm start mode %d", p->warm_start);
+  // GeometricImpactFrictionMap.cpp:258-262: "Caching in 3 space not yet supported."
+  if (p->warm_start == SCX_WARM_CACHE && c->dim == 3)
+    return fail(c, SCX_E_ARG, "normal and friction impulse caching is 2D only (use SCX_WARM_NORMAL)");
   return SCX_OK;
 }
 
@@ -428,6 +451,9 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     if (const char* env = getenv("SCX_PERSISTENT_PER_CU")) per_cu = std::min(4, std::max(1, atoi(env)));
     c->gs_blocks = std::min(solver_max_blocks(c), per_cu * c->num_cus);
     c->colour_blocks = std::min(colour_max_blocks(c), per_cu * c->num_cus);
+    // a fixed workgroup count for the persistent kernels, so that tests can run the
+    // device-wide barrier paths on small problems
+    if (const char* env = getenv("SCX_FORCE_PERSISTENT_BLOCKS")) c->force_blocks = std::max(0, atoi(env));
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
       break;
@@ -494,6 +520,10 @@ int scx_set_bodies(scx_ctx* c, uint32_t n, const double* q, const double* v, con
     HIPC(hipMemsetAsync(d.inertia, 0, sizeof(double) * n, s));
   }
   HIPC(hipStreamSynchronize(s));
+  // The impulse cache belongs to the scene, as the reference's constraint cache belongs to
+  // the simulation: new positions and velocities of the same bodies (a caller feeding
+  // each step's q0, v0) keep it, a different body count empties it.
+  if (n != c->n) d.prev_count = 0;
   c->n = n;
   c->has_bodies = true;
   c->order_valid = false;
@@ -642,11 +672,14 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   if (C > c->cap_contacts) {
     if (alloc_contacts(c, C) != SCX_OK || alloc_scan_tmp(c) != SCX_OK) return SCX_E_HIP;
   }
-  // the explicit problem replaces the resident scene
+  // the explicit problem replaces the resident scene; its bodies keep their own indices
   c->has_bodies = false;
   c->n = nbodies;
   hipStream_t s = c->stream;
   DeviceBuffers& d = c->d;
+  d.sid = nullptr;
+  d.slot_map = nullptr;
+  d.prev_count = 0;
   std::vector<double> Ivec(nbodies, 0.0), w0(3 * (size_t)nbodies, 0.0);
   if (D == 3) {
     for (uint32_t i = 0; i < nbodies; ++i) Ivec[i] = inertia ? inertia[i] : 0.0;
@@ -722,9 +755,10 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
     (void)nc;
     HIPC(launch_impulses(c, C));
   }
-  std::vector<double> bv(4 * (size_t)nbodies), bw(D == 3 ? 4 * (size_t)nbodies : 0);
+  // body records by slot (identity slots here): [v | minv] (2D), [v | minv][omega | Iinv] (3D)
+  const size_t rec = (D == 3) ? 8 : 4;
+  std::vector<double> bv(rec * (size_t)nbodies);
   HIPC(hipMemcpyAsync(bv.data(), d.bv64, sizeof(double) * bv.size(), hipMemcpyDeviceToHost, s));
-  if (D == 3) HIPC(hipMemcpyAsync(bw.data(), d.bw64, sizeof(double) * bw.size(), hipMemcpyDeviceToHost, s));
   if (C > 0) {
     HIPC(hipMemcpyAsync(alpha, d.calpha, sizeof(double) * C, hipMemcpyDeviceToHost, s));
     HIPC(hipMemcpyAsync(beta, d.cbeta, sizeof(double) * (D - 1) * C, hipMemcpyDeviceToHost, s));
@@ -734,9 +768,9 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   if (C > 0 && (d.ctr_host->overflow & 8u)) return fail(c, SCX_E_COLOURS, "contact graph needs more than 64 colours");
   for (uint32_t i = 0; i < nbodies; ++i) {
     if (v2)
-      for (int q = 0; q < D; ++q) v2[(size_t)D * i + q] = bv[4 * (size_t)i + q];
+      for (int q = 0; q < D; ++q) v2[(size_t)D * i + q] = bv[rec * i + q];
     if (omega2 && D == 3)
-      for (int q = 0; q < 3; ++q) omega2[3 * (size_t)i + q] = bw[4 * (size_t)i + q];
+      for (int q = 0; q < 3; ++q) omega2[3 * (size_t)i + q] = bv[rec * i + 4 + q];
   }
   // friction-only generalized impulse (Sobogus.cpp:335-346, :543-566): layout [lin | ang]
   if (f) {

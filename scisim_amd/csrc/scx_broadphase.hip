@@ -630,6 +630,21 @@ hipError_t launch_order_reset(scx_ctx* c, uint32_t n) {
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(kBlock) k_slots(uint32_t n, const uint32_t* __restrict__ order,
+                                                  uint32_t* __restrict__ slot_of) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < n) slot_of[order[s]] = s;
+}
+
+hipError_t launch_slots(scx_ctx* c, uint32_t n) {
+  const uint32_t nblk = (n + kBlock - 1) / kBlock;
+  c->d.sid = c->d.order;
+  c->d.slot_map = c->d.slot_of;
+  if (nblk == 0) return hipSuccess;
+  k_slots<<<nblk, kBlock, 0, c->stream>>>(n, c->d.order, c->d.slot_of);
+  return hipGetLastError();
+}
+
 // Home-cell sort (home_only) or full binning of the sweep-order boxes in d.tbox.
 // order_in == nullptr: the sweep order is the identity.
 template <int D>

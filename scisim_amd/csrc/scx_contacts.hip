@@ -329,8 +329,9 @@ hipError_t launch_colouring(scx_ctx* c, uint32_t C) {
   if (e != hipSuccess) return e;
   if (nblk) k_adj_fill<<<nblk, kBlock, 0, s>>>(C, c->d.ca, c->d.cb, deg, off, c->d.adj);
   if (C == 0) return hipGetLastError();
-  // small contact sets: one workgroup, whose rounds synchronise with workgroup barriers
-  const int blocks = (C <= 16 * kBlock) ? 1 : (int)std::min<uint32_t>((uint32_t)c->colour_blocks, nblk);
+  // every round passes over the uncoloured contacts between two barriers
+  const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(C, kBlock, c->colour_blocks);
+  const int blocks = std::max(1, std::min(c->colour_blocks, want));
   uint32_t Cv = C;
   const uint32_t* pa = c->d.ca;
   const uint32_t* pb = c->d.cb;
@@ -352,11 +353,66 @@ int colour_max_blocks(scx_ctx* c) {
 }
 
 // perm: colour order -> reference order (stable), coloff: colour offsets.
+// Contacts by the sweep slot of their first body (counting sort; contacts that share a
+// body, and so a slot, lie in different colours).
+__global__ void __launch_bounds__(kBlock) k_slot_count(uint32_t C, const uint32_t* __restrict__ ca,
+                                                       const uint32_t* __restrict__ slot_map,
+                                                       uint32_t* __restrict__ cnt) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < C) atomicAdd(&cnt[slot_map[ca[k]]], 1u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_slot_scatter(uint32_t C, const uint32_t* __restrict__ ca,
+                                                         const uint32_t* __restrict__ slot_map,
+                                                         const uint32_t* __restrict__ start,
+                                                         uint32_t* __restrict__ cnt, uint32_t* __restrict__ list) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t sl = slot_map[ca[k]];
+  list[start[sl] + atomicSub(&cnt[sl], 1u) - 1u] = k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_list_colour_masks(uint32_t C, const uint32_t* __restrict__ list,
+                                                              const int32_t* __restrict__ state,
+                                                              unsigned long long* __restrict__ mask) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j < C) mask[j] = 1ull << (state[list[j]] & 0xff);
+}
+
+__global__ void __launch_bounds__(kBlock) k_compose(uint32_t C, const uint32_t* __restrict__ list,
+                                                    uint32_t* __restrict__ perm) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < C) perm[k] = list[perm[k]];
+}
+
+// perm: colour order -> reference order, coloff: colour offsets. Within a colour the
+// contacts follow the sweep slot of their first body when slots are in use (the order
+// inside a colour does not change the sweep's result: its contacts share no body), else
+// the reference order.
 hipError_t launch_colour_sort(scx_ctx* c, uint32_t C) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
-  k_colour_masks<<<nblk, kBlock, 0, c->stream>>>(C, c->d.cstate, c->d.pmask);
-  return launch_multibucket(c, c->d.pmask, C, kMaxColours, c->d.perm, c->cap_perm, c->d.coloff);
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  if (!d.slot_map) {
+    k_colour_masks<<<nblk, kBlock, 0, s>>>(C, d.cstate, d.pmask);
+    return launch_multibucket(c, d.pmask, C, kMaxColours, d.perm, c->cap_perm, d.coloff);
+  }
+  const uint32_t n = c->n;
+  uint32_t* cnt = d.deg;              // n + 1 (free after the colouring)
+  uint32_t* start = d.deg + n + 1;    // n + 1
+  uint32_t* list = d.adj;             // C of 2C (free after the colouring)
+  hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (n + 1), s);
+  if (e != hipSuccess) return e;
+  k_slot_count<<<nblk, kBlock, 0, s>>>(C, d.ca, d.slot_map, cnt);
+  e = launch_exclusive_scan(cnt, start, n, d.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  k_slot_scatter<<<nblk, kBlock, 0, s>>>(C, d.ca, d.slot_map, start, cnt, list);
+  k_list_colour_masks<<<nblk, kBlock, 0, s>>>(C, list, d.cstate, d.pmask);
+  e = launch_multibucket(c, d.pmask, C, kMaxColours, d.perm, c->cap_perm, d.coloff);
+  if (e != hipSuccess) return e;
+  k_compose<<<nblk, kBlock, 0, s>>>(C, list, d.perm);
+  return hipGetLastError();
 }
 
 // Records in colour order for the sweeps (FP64, and FP32 when requested).
@@ -366,13 +422,16 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
                                                    const double* __restrict__ lb, const double* __restrict__ wn,
                                                    const double* __restrict__ wt, const double* __restrict__ forcing,
                                                    const double* __restrict__ rinit, double mu, const double* __restrict__ mu_arr,
-                                                   ContactSoA<double> o64,
+                                                   const uint32_t* __restrict__ slot_map, ContactSoA<double> o64,
                                                    ContactSoA<float> o32, int fp32) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= C) return;
   const uint32_t c = perm[k];
-  o64.a[k] = ca[c];
-  o64.b[k] = cb[c];
+  // the sweeps address bodies by slot
+  const uint32_t a = slot_map ? slot_map[ca[c]] : ca[c];
+  const uint32_t b = (cb[c] == kNoBody) ? kNoBody : (slot_map ? slot_map[cb[c]] : cb[c]);
+  o64.a[k] = a;
+  o64.b[k] = b;
   double n[3] = {0, 0, 0}, s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
   const int DD = dim * dim;
   for (int d = 0; d < dim; ++d) {
@@ -393,8 +452,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
   o64.mu[k] = muc;
   o64.forcing[k] = forcing[c];
   if (fp32) {
-    o32.a[k] = ca[c];
-    o32.b[k] = cb[c];
+    o32.a[k] = a;
+    o32.b[k] = b;
     for (int d = 0; d < 3; ++d) {
       o32.n[3 * (size_t)k + d] = (float)n[d];
       o32.s[3 * (size_t)k + d] = (float)s[d];
@@ -409,13 +468,164 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
   }
 }
 
+// Cache keys in active-set order, ascending: body pairs (i << 32 | j), then plane
+// contacts (1 << 63 | plane << 32 | i) -- the (i, j) and (plane, i) keys of
+// ball2d/ConstraintCache.cpp:20-123.
+__global__ void __launch_bounds__(kBlock) k_contact_keys(uint32_t C, const uint32_t* __restrict__ ca,
+                                                         const uint32_t* __restrict__ cb,
+                                                         const int32_t* __restrict__ cplane,
+                                                         unsigned long long* __restrict__ key) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  key[k] = (cb[k] != kNoBody) ? (((unsigned long long)ca[k] << 32) | cb[k])
+                              : ((1ull << 63) | ((unsigned long long)(uint32_t)cplane[k] << 32) | ca[k]);
+}
+
+// Cached world impulse r_old rotated by the minimal rotation taking the old normal n_old
+// to the current normal n (the quaternion FromTwoVectors / vector product that
+// SymplecticEulerImpactFrictionMap.cpp:96-115 applies); all-zero entries give zero. 2D
+// vectors rotate in the xy-plane.
+__device__ void transport_impulse(int dim, const double* n_old, const double* r_old, const double* n, double* f) {
+  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+  bool zero = true;
+  for (int k = 0; k < dim; ++k) {
+    a[k] = n_old[k];
+    b[k] = n[k];
+    r[k] = r_old[k];
+    zero = zero && n_old[k] == 0.0 && r_old[k] == 0.0;
+  }
+  f[0] = f[1] = f[2] = 0.0;
+  if (zero) return;
+  const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  const double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  for (int k = 0; k < 3; ++k) {
+    a[k] = a[k] / na;
+    b[k] = b[k] / nb;
+  }
+  const double c = b[0] * a[0] + b[1] * a[1] + b[2] * a[2];
+  double v[3], w;
+  if (c < -1.0 + 1e-12) {
+    // opposite normals: half turn about an axis orthogonal to the old normal
+    double ax[3];
+    orthogonal_vector(a, ax);
+    const double w2 = (1.0 + fmax(c, -1.0)) * 0.5;
+    w = sqrt(w2);
+    const double sv = sqrt(1.0 - w2);
+    for (int k = 0; k < 3; ++k) v[k] = ax[k] * sv;
+  } else {
+    const double ax[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double s = sqrt((1.0 + c) * 2.0);
+    const double invs = 1.0 / s;
+    for (int k = 0; k < 3; ++k) v[k] = ax[k] * invs;
+    w = s * 0.5;
+  }
+  double uv[3] = {v[1] * r[2] - v[2] * r[1], v[2] * r[0] - v[0] * r[2], v[0] * r[1] - v[1] * r[0]};
+  for (int k = 0; k < 3; ++k) uv[k] = uv[k] + uv[k];
+  const double vu[3] = {v[1] * uv[2] - v[2] * uv[1], v[2] * uv[0] - v[0] * uv[2], v[0] * uv[1] - v[1] * uv[0]};
+  for (int k = 0; k < 3; ++k) f[k] = (r[k] + w * uv[k]) + vu[k];
+}
+
+// initializeImpulses (GeometricImpactFrictionMap.cpp:174-220): the cached impulse of the
+// same key (binary search in the previous step's ascending keys), zero on a miss
+// (ball2d/ConstraintCache.cpp:122). World impulse alpha n + sum_k beta_k t_k
+// (Sobogus.cpp:301, :518; beta = 0 for NORMAL), or the transported world impulse, then
+// local coordinates E^T r. Cache entries are kCacheWords doubles: (alpha, beta...) or
+// (old normal, world impulse).
+constexpr int kCacheWords = 6;
+
+__global__ void __launch_bounds__(kBlock) k_warm_lookup(uint32_t C, int dim, int mode,
+                                                        const unsigned long long* __restrict__ key,
+                                                        const unsigned long long* __restrict__ pkey, uint32_t P,
+                                                        const double* __restrict__ pr,
+                                                        const double* __restrict__ basis,
+                                                        double* __restrict__ rinit) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const unsigned long long kk = key[k];
+  uint32_t lo = 0, hi = P;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pkey[mid] < kk)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  double r[3] = {0.0, 0.0, 0.0};
+  if (lo < P && pkey[lo] == kk) {
+    const double* B = basis + (size_t)dim * dim * k;
+    const double* c = pr + (size_t)kCacheWords * lo;
+    double rw[3] = {0.0, 0.0, 0.0};
+    if (mode == SCX_WARM_TRANSPORT) {
+      transport_impulse(dim, c, c + 3, B, rw);
+    } else {
+      for (int a = 0; a < dim; ++a) rw[a] = c[0] * B[a];
+      if (mode == SCX_WARM_CACHE)
+        for (int j = 1; j < dim; ++j)
+          for (int a = 0; a < dim; ++a) rw[a] = rw[a] + c[j] * B[j * dim + a];
+    }
+    for (int j = 0; j < dim; ++j) {
+      double acc = B[j * dim] * rw[0];
+      for (int a = 1; a < dim; ++a) acc = acc + B[j * dim + a] * rw[a];
+      r[j] = acc;
+    }
+  }
+  for (int j = 0; j < dim; ++j) rinit[(size_t)dim * k + j] = r[j];
+}
+
+// cacheImpulses (GeometricImpactFrictionMap.cpp:222-264): alpha, and beta with friction;
+// for the transported cache the normal and the world impulse alpha n + sum_k beta_k t_k.
+__global__ void __launch_bounds__(kBlock) k_cache_store(uint32_t C, int dim, int mode,
+                                                        const unsigned long long* __restrict__ key,
+                                                        const double* __restrict__ basis,
+                                                        const double* __restrict__ alpha,
+                                                        const double* __restrict__ beta,
+                                                        unsigned long long* __restrict__ pkey,
+                                                        double* __restrict__ pr) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  pkey[k] = key[k];
+  double* o = pr + (size_t)kCacheWords * k;
+  for (int j = 0; j < kCacheWords; ++j) o[j] = 0.0;
+  if (mode == SCX_WARM_TRANSPORT) {
+    const double* B = basis + (size_t)dim * dim * k;
+    for (int a = 0; a < dim; ++a) {
+      o[a] = B[a];
+      double w = alpha[k] * B[a];
+      for (int j = 1; j < dim; ++j) w = w + beta[(size_t)(dim - 1) * k + j - 1] * B[j * dim + a];
+      o[3 + a] = w;
+    }
+  } else {
+    o[0] = alpha[k];
+    if (mode == SCX_WARM_CACHE)
+      for (int j = 1; j < dim; ++j) o[j] = beta[(size_t)(dim - 1) * k + j - 1];
+  }
+}
+
+hipError_t launch_warm_start(scx_ctx* c, uint32_t C, int mode) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  k_contact_keys<<<nblk, kBlock, 0, c->stream>>>(C, c->d.ca, c->d.cb, c->d.cplane, c->d.cur_key);
+  k_warm_lookup<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mode, c->d.cur_key, c->d.prev_key, c->d.prev_count,
+                                                c->d.prev_r, c->d.basis, c->d.crinit);
+  return hipGetLastError();
+}
+
+// The cache must hold C entries of kCacheWords (the caller grows it).
+hipError_t launch_cache_store(scx_ctx* c, uint32_t C, int mode) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  k_cache_store<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mode, c->d.cur_key, c->d.basis, c->d.calpha,
+                                                c->d.cbeta, c->d.prev_key, c->d.prev_r);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* mu_arr,
                                  const double* rinit) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
   k_gather<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, c->d.perm, c->d.ca, c->d.cb, c->d.basis, c->d.cla, c->d.clb,
-                                           c->d.cwn, c->d.cwt, c->d.cforcing, rinit, mu, mu_arr, c->d.c64, c->d.c32,
-                                           fp32 ? 1 : 0);
+                                           c->d.cwn, c->d.cwt, c->d.cforcing, rinit, mu, mu_arr, c->d.slot_map,
+                                           c->d.c64, c->d.c32, fp32 ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -77,11 +77,17 @@ struct DeviceBuffers {
   double* q1p = nullptr;     // predictor positions
   double* v2 = nullptr;      // post-impact velocity (dim*N)
   double* w2 = nullptr;      // (3D)
-  // working velocities for the sweeps: per body [v (3) | minv] and [omega (3) | Iinv]
-  double* bv64 = nullptr;    // 4*N
-  double* bw64 = nullptr;    // 4*N
-  float* bv32 = nullptr;
-  float* bw32 = nullptr;
+  // working velocities for the sweeps, per body slot: 2D [v (3) | minv], 3D
+  // [v (3) | minv][omega (3) | Iinv] adjacent, so that a body is one or two vector loads
+  // from one cache line
+  double* bv64 = nullptr;    // 8*N
+  float* bv32 = nullptr;     // 8*N
+  // Body slots of the sweeps: slot_of[id] = slot, sid[slot] = id. A step uses the
+  // broad phase's home-cell order (spatially sorted), so that the bodies of neighbouring
+  // contacts sit in neighbouring memory; scx_friction_solve uses the identity.
+  uint32_t* slot_of = nullptr;  // N
+  const uint32_t* sid = nullptr;        // slot -> id of the current solve, nullptr = identity
+  const uint32_t* slot_map = nullptr;   // id -> slot of the current solve, nullptr = identity
   // planes
   double* px = nullptr;      // dim*P
   double* pn = nullptr;      // dim*P (normalised)
@@ -132,10 +138,13 @@ struct DeviceBuffers {
   double* rbest = nullptr;     // 3 per contact
   double* partial = nullptr;   // per-block residual partials
   // warm-start cache (previous step's active set, sorted keys + impulses)
-  unsigned long long* prev_key = nullptr;
-  double* prev_r = nullptr;    // dim per contact, local
+  // keys ascend in active-set order: (i << 32 | j) for body pairs, then
+  // (1 << 63 | plane << 32 | i) for plane contacts
+  unsigned long long* prev_key = nullptr;  // cap_cache
+  double* prev_r = nullptr;    // cached (alpha, beta...) per key, dim each (cap_cache)
   uint32_t prev_count = 0;
-  unsigned long long* cur_key = nullptr;
+  uint32_t cap_cache = 0;
+  unsigned long long* cur_key = nullptr;   // this step's keys (cap_contacts)
   StepCounters* ctr = nullptr;
   uint32_t* barrier = nullptr;       // grid barrier state of the persistent kernels (zeroed once)
   StepCounters* ctr_host = nullptr;  // pinned
@@ -163,6 +172,7 @@ struct scx_ctx {
   std::vector<const char*> phase_names;
   int gs_blocks = 0;
   int colour_blocks = 0;
+  int force_blocks = 0;  // > 0: persistent kernels use this many workgroups (capped), for tests
   scx::DeviceBuffers d;
 };
 
@@ -181,6 +191,8 @@ hipError_t launch_flow(scx_ctx* c, uint32_t n, int integrator, const double* m, 
 // otherwise the predictor has written the sweep-order records of this step.
 hipError_t launch_grid_build(scx_ctx* c, uint32_t n, bool given_boxes);
 hipError_t launch_order_reset(scx_ctx* c, uint32_t n);
+// Body slots of the sweeps from the sweep order in d.order (d.sid, d.slot_map).
+hipError_t launch_slots(scx_ctx* c, uint32_t n);
 hipError_t launch_pairs(scx_ctx* c, uint32_t n, int mode, int filter, uint32_t* out_a, uint32_t* out_b, uint32_t cap);
 hipError_t launch_plane_masks(scx_ctx* c, uint32_t n);
 hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint32_t n, uint32_t nbuckets,
@@ -193,6 +205,10 @@ hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, c
 hipError_t launch_solve(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p);
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p);
 hipError_t launch_impulses(scx_ctx* c, uint32_t C);
+// Impulse cache: keys of this step's active set and the warm start from the previous
+// step's cache into d.crinit; then this step's alpha/beta into the cache.
+hipError_t launch_warm_start(scx_ctx* c, uint32_t C, int mode);
+hipError_t launch_cache_store(scx_ctx* c, uint32_t C, int mode);
 int solver_max_blocks(scx_ctx* c);
 int colour_max_blocks(scx_ctx* c);
 }  // namespace scx

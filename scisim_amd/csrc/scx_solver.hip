@@ -158,11 +158,16 @@ __device__ __forceinline__ double fb_err(double mu, const double* r, const doubl
   return e;
 }
 
+// Body records per slot: 2D one vector [v (3) | minv], 3D two adjacent vectors
+// [v (3) | minv][omega (3) | Iinv].
+template <int D>
+constexpr int kBodyVecs = (D == 3) ? 2 : 1;
+
 template <typename Real>
 struct GSArgs {
   ContactSoA<Real> c;
-  Real* bv;                 // 4 per body: v (3), minv
-  Real* bw;                 // 4 per body: omega (3), Iinv
+  Real* body;               // body records by slot
+  const uint32_t* sid;      // slot -> body id (nullptr: identity)
   const double* v0;         // dim per body
   const double* w0;         // 3 per body (3D)
   uint32_t nbodies;
@@ -185,13 +190,14 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
   Rec<D, Real> R;
   load_rec<D, Real>(A.c, k, R);
   const bool hasb = R.b != kNoBody;
-  V* bv = reinterpret_cast<V*>(A.bv);
-  V* bw = reinterpret_cast<V*>(A.bw);
-  V va = bv[R.a], wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
-  if (D == 3) wa = bw[R.a];
+  V* bv = reinterpret_cast<V*>(A.body);
+  V* bw = bv + 1;  // 3D: the angular half of the same record
+  const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
+  V va = bv[ia], wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
+  if (D == 3) wa = bw[ia];
   if (hasb) {
-    vb = bv[R.b];
-    if (D == 3) wb = bw[R.b];
+    vb = bv[ib];
+    if (D == 3) wb = bw[ib];
   }
   Real* rp = A.c.r + 3 * (size_t)k;
   const Real r[3] = {rp[0], rp[1], rp[2]};
@@ -217,7 +223,7 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
   va.x += va.w * P[0];
   va.y += va.w * P[1];
   va.z += va.w * P[2];
-  bv[R.a] = va;
+  bv[ia] = va;
   Real tau[3];
   if (D == 3) {
 #pragma unroll
@@ -226,19 +232,19 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
     wa.x += ka * tau[0];
     wa.y += ka * tau[1];
     wa.z += ka * tau[2];
-    bw[R.a] = wa;
+    bw[ia] = wa;
   }
   if (hasb) {
     vb.x -= vb.w * P[0];
     vb.y -= vb.w * P[1];
     vb.z -= vb.w * P[2];
-    bv[R.b] = vb;
+    bv[ib] = vb;
     if (D == 3) {
       const Real kb = R.lb * wb.w;
       wb.x -= kb * tau[0];
       wb.y -= kb * tau[1];
       wb.z -= kb * tau[2];
-      bw[R.b] = wb;
+      bw[ib] = wb;
     }
   }
 }
@@ -248,13 +254,14 @@ __device__ __forceinline__ double contact_err(const GSArgs<Real>& A, uint32_t k)
   using V = typename Vec4<Real>::type;
   Rec<D, Real> R;
   load_rec<D, Real>(A.c, k, R);
-  const V* bv = reinterpret_cast<const V*>(A.bv);
-  const V* bw = reinterpret_cast<const V*>(A.bw);
+  const V* bv = reinterpret_cast<const V*>(A.body);
+  const V* bw = bv + 1;
+  const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
   const V z = {0, 0, 0, 0};
-  const V va = bv[R.a];
-  const V wa = (D == 3) ? bw[R.a] : z;
-  const V vb = (R.b != kNoBody) ? bv[R.b] : z;
-  const V wb = (D == 3 && R.b != kNoBody) ? bw[R.b] : z;
+  const V va = bv[ia];
+  const V wa = (D == 3) ? bw[ia] : z;
+  const V vb = (R.b != kNoBody) ? bv[ib] : z;
+  const V wb = (D == 3 && R.b != kNoBody) ? bw[ib] : z;
   double u[3];
   rel_vel<D, double, Real>(R, va, wa, vb, wb, u);
   const Real* rp = A.c.r + 3 * (size_t)k;
@@ -299,13 +306,15 @@ __device__ void apply_impulses(const GSArgs<Real>& A, const uint32_t* off, uint3
     for (uint32_t k = gtid; k < A.C; k += gsize) {
       const uint32_t bodies[2] = {A.c.a[k], A.c.b[k]};
       for (int q = 0; q < 2; ++q) {
-        const uint32_t i = bodies[q];
-        if (i == kNoBody) continue;
+        const uint32_t sl = bodies[q];
+        if (sl == kNoBody) continue;
+        const uint32_t i = A.sid ? A.sid[sl] : sl;
+        Real* rec = A.body + 4 * kBodyVecs<D> * (size_t)sl;
 #pragma unroll
-        for (int d = 0; d < D; ++d) A.bv[4 * (size_t)i + d] = (Real)A.v0[(size_t)D * i + d];
+        for (int d = 0; d < D; ++d) rec[d] = (Real)A.v0[(size_t)D * i + d];
         if (D == 3)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) A.bw[4 * (size_t)i + d] = (Real)A.w0[3 * (size_t)i + d];
+          for (int d = 0; d < 3; ++d) rec[4 + d] = (Real)A.w0[3 * (size_t)i + d];
       }
     }
     grid_barrier(A.bar);
@@ -325,8 +334,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   for (uint32_t c = threadIdx.x; c <= ncol; c += kGSBlock) off[c] = A.coloff[c];
   __syncthreads();
   uint32_t max_iters = A.max_iters;
-  // FP32 phase: at most half the sweep budget, the FP64 polish gets the rest
-  if (A.phase == 1) max_iters = max(1u, max_iters / 2);
+  // FP32 phase: at most three quarters of the sweep budget, the FP64 polish gets the rest
+  if (A.phase == 1) max_iters = max(1u, max_iters - max_iters / 4);
   if (A.phase == 2) max_iters = (A.ctr->sweeps_fp32 < max_iters) ? max_iters - A.ctr->sweeps_fp32 : 0u;
   const bool track_best = A.phase != 1;
 
@@ -340,7 +349,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
     grid_barrier(A.bar);
   }
-  // FP32 phase stagnation: leave when 64 sweeps did not halve the best residual
+  // FP32 phase stagnation: leave when 256 sweeps lowered the best residual by less than
+  // 1% (FP32 has reached its rounding floor; FP64 continues from there)
   double ref = best;
   uint32_t ref_it = 0;
   uint32_t it = 0;
@@ -363,8 +373,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
           }
         }
         if (err < A.tol) break;
-        if (A.phase == 1 && it - ref_it >= 64) {
-          if (best > 0.5 * ref) {
+        if (A.phase == 1 && it - ref_it >= 256) {
+          if (best > 0.99 * ref) {
             stagnated = true;
             break;
           }
@@ -394,15 +404,18 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   }
 }
 
-// Per-body packed [v | minv], [omega | Iinv] in both precisions.
+// Body records [v | minv] ([omega | Iinv]) by slot, in both precisions.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_body_init(uint32_t n, const double* __restrict__ v0,
-                                                      const double* __restrict__ w0, const double* __restrict__ m,
-                                                      const double* __restrict__ inertia, double* __restrict__ bv64,
-                                                      double* __restrict__ bw64, float* __restrict__ bv32,
-                                                      float* __restrict__ bw32, int fp32) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+__global__ void __launch_bounds__(kBlock) k_body_init(uint32_t n, const uint32_t* __restrict__ sid,
+                                                      const double* __restrict__ v0, const double* __restrict__ w0,
+                                                      const double* __restrict__ m,
+                                                      const double* __restrict__ inertia, double* __restrict__ body64,
+                                                      float* __restrict__ body32, int fp32) {
+  const uint32_t sl = blockIdx.x * kBlock + threadIdx.x;
+  if (sl >= n) return;
+  const uint32_t i = sid ? sid[sl] : sl;
+  double4* bv64 = reinterpret_cast<double4*>(body64) + (size_t)kBodyVecs<D> * sl;
+  float4* bv32 = reinterpret_cast<float4*>(body32) + (size_t)kBodyVecs<D> * sl;
   double v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < D; ++d) v[d] = v0[(size_t)D * i + d];
@@ -411,11 +424,11 @@ __global__ void __launch_bounds__(kBlock) k_body_init(uint32_t n, const double* 
     for (int d = 0; d < 3; ++d) w[d] = w0[3 * (size_t)i + d];
   const double minv = 1.0 / m[i];
   const double iinv = (D == 3) ? 1.0 / inertia[i] : 0.0;
-  reinterpret_cast<double4*>(bv64)[i] = make_double4(v[0], v[1], v[2], minv);
-  if (D == 3) reinterpret_cast<double4*>(bw64)[i] = make_double4(w[0], w[1], w[2], iinv);
+  bv64[0] = make_double4(v[0], v[1], v[2], minv);
+  if (D == 3) bv64[1] = make_double4(w[0], w[1], w[2], iinv);
   if (fp32) {
-    reinterpret_cast<float4*>(bv32)[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)minv);
-    if (D == 3) reinterpret_cast<float4*>(bw32)[i] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)iinv);
+    bv32[0] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)minv);
+    if (D == 3) bv32[1] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)iinv);
   }
 }
 
@@ -429,8 +442,8 @@ __global__ void __launch_bounds__(kBlock) k_promote_r(uint32_t C, const float* _
 // End of step: v2 from the sweep velocities, impulses back to the reference order,
 // and the final flow from q0 with v2 (GeometricImpactFrictionMap.cpp:441).
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_finish_bodies(uint32_t n, const double* __restrict__ bv64,
-                                                          const double* __restrict__ bw64,
+__global__ void __launch_bounds__(kBlock) k_finish_bodies(uint32_t n, const uint32_t* __restrict__ slot_of,
+                                                          const double* __restrict__ body64,
                                                           const double* __restrict__ q0, const double* __restrict__ m,
                                                           double dt, double gx, double gy, double gz, int integrator,
                                                           double* __restrict__ q, double* __restrict__ v,
@@ -439,7 +452,8 @@ __global__ void __launch_bounds__(kBlock) k_finish_bodies(uint32_t n, const doub
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const double g[3] = {gx, gy, gz};
-  const double4 bvi = reinterpret_cast<const double4*>(bv64)[i];
+  const double4* rec = reinterpret_cast<const double4*>(body64) + (size_t)kBodyVecs<D> * (slot_of ? slot_of[i] : i);
+  const double4 bvi = rec[0];
   const double vv[3] = {bvi.x, bvi.y, bvi.z};
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -448,7 +462,7 @@ __global__ void __launch_bounds__(kBlock) k_finish_bodies(uint32_t n, const doub
     flow_component<D>(integrator, m[i], g[d], dt, q0[k], vv[d], &q[k], &v[k]);
   }
   if (D == 3) {
-    const double4 bwi = reinterpret_cast<const double4*>(bw64)[i];
+    const double4 bwi = rec[1];
     w2[3 * (size_t)i] = w[3 * (size_t)i] = bwi.x;
     w2[3 * (size_t)i + 1] = w[3 * (size_t)i + 1] = bwi.y;
     w2[3 * (size_t)i + 2] = w[3 * (size_t)i + 2] = bwi.z;
@@ -465,19 +479,13 @@ __global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, co
   for (int d = 1; d < dim; ++d) beta[(size_t)(dim - 1) * c + d - 1] = (mu == 0.0) ? 0.0 : r64[3 * (size_t)k + d];
 }
 
-// Persistent launch sized to the largest colour: one contact per thread per colour
-// pass, so the barrier (whose cost grows with the number of workgroups) spans no
-// more workgroups than the widest pass needs.
-// Up to kGSSingleMax contacts per colour one workgroup sweeps them in a few passes:
-// its barrier is a workgroup barrier (~0.1 us) instead of a device-wide one (several us
-// across XCDs), which wins while the colours are a few thousand contacts wide.
-constexpr uint32_t kGSSingleMax = 8 * kGSBlock;
-
+// Persistent launch sized by the widest colour and the barrier/pass trade-off of
+// persistent_blocks (the barrier's cost grows with the workgroups): one workgroup (a
+// workgroup barrier, no device fence) for narrow colours, more as colours widen.
 template <int D, typename Real>
 static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, uint32_t max_colour) {
-  const uint32_t want =
-      (max_colour <= kGSSingleMax) ? 1u : (std::max<uint32_t>(max_colour, 1u) + kGSBlock - 1) / kGSBlock;
-  const int blocks = (int)std::max<uint32_t>(1u, std::min<uint32_t>((uint32_t)max_blocks, want));
+  const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(max_colour, kGSBlock, max_blocks);
+  const int blocks = std::max(1, std::min(max_blocks, want));
   void* args[] = {&A};
   return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
 }
@@ -498,13 +506,13 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
   const bool warm = p->warm_start != SCX_WARM_NONE;
   const uint32_t nb = (c->n + kBlock - 1) / kBlock;
   if (nb)
-    k_body_init<D><<<nb, kBlock, 0, s>>>(c->n, c->d.v0, c->d.w0, c->d.m, c->d.inertia, c->d.bv64, c->d.bw64,
-                                          c->d.bv32, c->d.bw32, fp32 ? 1 : 0);
+    k_body_init<D><<<nb, kBlock, 0, s>>>(c->n, c->d.sid, c->d.v0, c->d.w0, c->d.m, c->d.inertia, c->d.bv64,
+                                          c->d.bv32, fp32 ? 1 : 0);
   hipError_t e = hipGetLastError();
   timing_mark(c, "gs_init");
   if (e != hipSuccess || C == 0) return e;
   if (fp32) {
-    GSArgs<float> A{c->d.c32, c->d.bv32, c->d.bw32, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
+    GSArgs<float> A{c->d.c32, c->d.bv32, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
                     reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
     e = coop_gs<D, float>(c, A, c->gs_blocks, max_colour);
@@ -516,7 +524,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
     if (e != hipSuccess) return e;
     timing_mark(c, "gs_promote");
   }
-  GSArgs<double> B{c->d.c64, c->d.bv64, c->d.bw64, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
+  GSArgs<double> B{c->d.c64, c->d.bv64, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
   return coop_gs<D, double>(c, B, c->gs_blocks, max_colour);
@@ -531,10 +539,10 @@ hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {
   const uint32_t nb = (c->n + kBlock - 1) / kBlock;
   if (nb) {
     if (c->dim == 2)
-      k_finish_bodies<2><<<nb, kBlock, 0, s>>>(c->n, c->d.bv64, c->d.bw64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
+      k_finish_bodies<2><<<nb, kBlock, 0, s>>>(c->n, c->d.slot_map, c->d.bv64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
                                                p->g[2], p->integrator, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
     else
-      k_finish_bodies<3><<<nb, kBlock, 0, s>>>(c->n, c->d.bv64, c->d.bw64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
+      k_finish_bodies<3><<<nb, kBlock, 0, s>>>(c->n, c->d.slot_map, c->d.bv64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
                                                p->g[2], p->integrator, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
   }
   const uint32_t nc = (C + kBlock - 1) / kBlock;

@@ -239,3 +239,90 @@ def test_empty_active_set_keeps_predictor(ctx2):
     q1, v1, _ = ctx2.get_state()
     oq, ov = oracle.umap_flow(2, 0, np.ones(n), [0.0, -9.81, 0.0], 0.01, q, v)
     assert np.array_equal(q1, oq) and np.array_equal(v1, ov)
+
+
+@pytest.mark.parametrize("dim,mode", [(2, scx.WARM_CACHE), (2, scx.WARM_NORMAL), (3, scx.WARM_NORMAL),
+                                      (3, scx.WARM_TRANSPORT)])
+def test_impulse_cache_warm_start_matches_oracle(dim, mode):
+    """The impulse cache (ImpulsesToCache) carried between steps on the device gives the
+    oracle's warm starts. At a tolerance neither side reaches (fixed sweep count) the
+    impulses agree to rounding; at 1e-6 the warm-started solves stop within two sweeps of
+    the oracle's and take fewer sweeps than cold starts from the same states."""
+    ctx = scx.Context(dim, 40000, max_planes=8)  # its own, empty cache
+    if dim == 2:
+        s = scenes.ball2d_lattice(m=48, side=7.5, r=0.05, vstd=0.1, seed=5)
+        sim = oracle.Sim(2, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"])
+        prm = _params(2, dt=0.01, mu=0.5, tol=1e-10, max_iters=2000, warm_start=mode)
+        warmup = 60
+    else:
+        s = _spheres_in_box(3000, 4)
+        sim = oracle.Sim(3, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"], inertia=s["inertia"])
+        prm = _params(3, dt=0.005, mu=0.4, tol=1e-10, max_iters=2000, warm_start=mode)
+        warmup = 80
+    ctx.set_planes(s["px"], s["pn"])
+    sim.run(warmup, **dict(prm, warm_start=0))
+    # one step on each side fills both caches from the same state
+    _compare_step(ctx, sim, prm, tag="warm fill")
+    worst = {}
+    for k in range(3):
+        e = _compare_step(ctx, sim, prm, tag="warm %dd step %d" % (dim, k))
+        for key in ("alpha", "beta", "v", "q"):
+            worst[key] = max(worst.get(key, 0.0), e[key])
+        assert abs(e["sweeps"][0] - e["sweeps"][1]) <= 1
+    _say("warm", dim, mode, worst)
+    assert worst["alpha"] < 1e-8 and worst["beta"] < 1e-8 and worst["v"] < 1e-8 and worst["q"] < 1e-11
+    conv = dict(prm, tol=1e-6, max_iters=5000)
+    cold = dict(conv, warm_start=0)
+    warm_sweeps, oracle_sweeps, cold_sweeps = [], [], []
+    for k in range(4):
+        probe = oracle.Sim(dim, sim.q, sim.v, sim.r, sim.m, sim.px, sim.pn, w=sim.w, inertia=sim.inertia)
+        cold_sweeps.append(probe.step(order=1, **cold)["sweeps"])
+        e = _compare_step(ctx, sim, conv, tag="warm %dd tol 1e-6 step %d" % (dim, k))
+        warm_sweeps.append(e["sweeps"][0])
+        oracle_sweeps.append(e["sweeps"][1])
+    _say("sweeps warm", warm_sweeps, "oracle warm", oracle_sweeps, "cold", cold_sweeps)
+    ctx.close()
+    assert all(abs(a - b) <= 2 for a, b in zip(warm_sweeps, oracle_sweeps))
+    assert sum(warm_sweeps) < sum(cold_sweeps)
+
+
+def test_normal_and_friction_cache_is_rejected_in_3d(ctx3):
+    s = _spheres_in_box(200, 1)
+    ctx3.set_planes(s["px"], s["pn"])
+    ctx3.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s["inertia"])
+    with pytest.raises(scx.ScxError):
+        ctx3.step(scx.step_params(**_params(3, warm_start=scx.WARM_CACHE)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_multi_workgroup_barrier_paths_match_oracle(monkeypatch, dim):
+    """The persistent colouring and sweep kernels on 16 workgroups (device-wide barriers
+    between colours and rounds) give the oracle's colours and rounding-level impulses, as
+    the single-workgroup launches that small problems get by default."""
+    monkeypatch.setenv("SCX_FORCE_PERSISTENT_BLOCKS", "16")
+    ctx = scx.Context(dim, 40000, max_planes=8)
+    if dim == 2:
+        s = scenes.ball2d_lattice(m=48, side=7.5, r=0.05, vstd=0.1, seed=11)
+        sim = oracle.Sim(2, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"])
+        prm = _params(2, dt=0.01, mu=0.5, max_iters=1000)
+        warmup = 60
+    else:
+        s = _spheres_in_box(3000, 6)
+        sim = oracle.Sim(3, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"], inertia=s["inertia"])
+        prm = _params(3, dt=0.005, mu=0.4, max_iters=1000)
+        warmup = 80
+    ctx.set_planes(s["px"], s["pn"])
+    sim.run(warmup, **prm)
+    worst = {}
+    for k in range(3):
+        e = _compare_step(ctx, sim, prm, tag="16 workgroups %dd step %d" % (dim, k))
+        for key in ("alpha", "beta", "v", "q"):
+            worst[key] = max(worst.get(key, 0.0), e[key])
+        assert abs(e["sweeps"][0] - e["sweeps"][1]) <= 1
+    # FP32 sweeps + FP64 polish on the same launch shape
+    ctx.set_bodies(sim.q, sim.v, sim.r, sim.m, omega=sim.w, inertia=sim.inertia)
+    st = ctx.step(scx.step_params(precision=scx.FP32_POLISH, **dict(prm, tol=1e-6, max_iters=5000)))
+    ctx.close()
+    _say("16 workgroups", dim, worst, "fp32+polish", st)
+    assert worst["alpha"] < 1e-8 and worst["beta"] < 1e-8 and worst["v"] < 1e-8 and worst["q"] < 1e-11
+    assert st["ncontacts"] > 0 and np.isfinite(st["error"])

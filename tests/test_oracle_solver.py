@@ -236,3 +236,67 @@ def test_integrators_closed_form(dim, integrator):
         else:
             # Verlet is exact for constant acceleration
             assert np.allclose(q, q0 + v0 * t + 0.5 * g * t * t, atol=1e-11)
+
+
+# ----------------------------------------------------------------------------- impulse cache
+@pytest.mark.parametrize("dim,mode", [(2, 1), (2, 2), (3, 2), (2, 3), (3, 3)])
+def test_cached_impulses_warm_start_a_resting_body(dim, mode):
+    """A body resting on the floor receives alpha = m g dt every step (the geometric map
+    solves on v0 = -g dt); with the previous step's impulse as warm start the solve
+    starts at the solution and needs no sweep, while a cold start needs at least one."""
+    g = [0.0, -9.81, 0.0][:dim]
+    dt, m, rad = 0.01, 2.0, 0.1
+    q = np.zeros(dim)
+    q[1] = rad
+    planes_x, planes_n = np.zeros(dim), np.eye(dim)[1]
+    out = {}
+    for warm in (0, mode):
+        sim = oracle.Sim(dim, q, np.zeros(dim), [rad], [m], planes_x, planes_n)
+        stats = [sim.step(dt=dt, g=g, mu=0.5, tol=1e-12, warm_start=warm) for _ in range(4)]
+        out[warm] = (stats, sim.contacts()["alpha"])
+    for warm, (stats, alpha) in out.items():
+        assert alpha.size == 1 and abs(alpha[0] - m * 9.81 * dt) < 1e-12
+    assert all(s["sweeps"] >= 1 for s in out[0][0][1:])
+    assert all(s["sweeps"] == 0 for s in out[mode][0][2:])
+
+
+def test_normal_and_friction_cache_is_2d_only():
+    sim = oracle.Sim(3, [0, 0.1, 0], np.zeros(3), [0.1], [1.0], np.zeros(3), [0, 1.0, 0])
+    with pytest.raises(RuntimeError):
+        sim.step(dt=0.01, g=[0, -9.81, 0], mu=0.5, warm_start=1)
+
+
+def _frame(n, rng):
+    n = n / np.linalg.norm(n)
+    if n.size == 2:
+        return np.c_[n, [-n[1], n[0]]]
+    s = np.cross(n, rng.normal(size=3))
+    s /= np.linalg.norm(s)
+    return np.c_[n, s, np.cross(n, s)]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_transported_impulse_is_the_rotation_taking_old_to_new_normal(dim):
+    """The transported cache rotates the old world impulse with the normal: lengths are
+    kept, the old normal lands on the new one (so a purely normal impulse stays purely
+    normal), tangential stays tangential, the angle to the normal is kept, zero stays zero,
+    and opposite normals still give a proper rotation."""
+    rng = np.random.default_rng(dim)
+    for _ in range(50):
+        n_old = rng.normal(size=dim)
+        n_old /= np.linalg.norm(n_old)
+        B = _frame(rng.normal(size=dim), rng)
+        r_old = rng.normal(size=dim)
+        loc = oracle.transport(n_old, r_old, B)
+        f = B @ loc
+        assert abs(np.linalg.norm(f) - np.linalg.norm(r_old)) < 1e-9
+        assert abs(f @ B[:, 0] - r_old @ n_old) < 1e-9
+        pure = oracle.transport(n_old, 2.5 * n_old, B)
+        assert abs(pure[0] - 2.5) < 1e-9 and np.abs(pure[1:]).max() < 1e-9
+        tang = r_old - (r_old @ n_old) * n_old
+        assert abs(oracle.transport(n_old, tang, B)[0]) < 1e-9
+    assert np.all(oracle.transport(np.zeros(dim), np.zeros(dim), _frame(np.eye(dim)[0], rng)) == 0.0)
+    B = _frame(np.eye(dim)[1], rng)
+    flip = oracle.transport(-np.eye(dim)[1], np.r_[0.3, -1.0, 0.2][:dim], B)
+    assert abs(np.linalg.norm(flip) - np.linalg.norm(np.r_[0.3, -1.0, 0.2][:dim])) < 1e-9
+    assert abs(flip[0] - 1.0) < 1e-9  # the old normal component follows the normal
