@@ -57,6 +57,13 @@ extern "C" {
 #define SCX_WARM_NORMAL 2    /* ImpulsesToCache::NORMAL: alpha only (2D and 3D) */
 #define SCX_WARM_TRANSPORT 3 /* world impulse rotated with the contact normal, then projected on the
                                 new basis (the 3D cache of SymplecticEulerImpactFrictionMap.cpp:87-127) */
+#define SCX_WARM_LEARNED 4   /* impulses predicted per contact by a 16-128-128-3 BF16 perceptron from
+                                the contact's features and its transported cached impulse; no
+                                counterpart in the reference (the solve still runs to tolerance) */
+
+/* Parameters of the warm-start perceptron (FP32): W1 [16][128], b1 [128], W2 [128][128],
+ * b2 [128], W3 [128][3], b3 [3], row-major, y = W3^T relu(W2^T relu(W1^T x + b1) + b2) + b3. */
+#define SCX_MLP_PARAMS 19075
 
 typedef struct scx_ctx scx_ctx;
 
@@ -165,6 +172,25 @@ int scx_friction_solve(scx_ctx* ctx, uint32_t nbodies, const double* m, const do
                        const double* omega0, const scx_contact_set* contacts, double tol, uint32_t max_iters,
                        uint32_t eval_every, int32_t precision, double* alpha, double* beta, double* v2,
                        double* omega2, double* f, int32_t* succeeded, double* error, uint32_t* sweeps);
+
+/* Learned warm start (SCX_WARM_LEARNED); no counterpart in the reference.
+ * A sample is one contact: 16 features (normal; pre-solve relative velocity in the contact
+ * frame over |g| dt; w_t / w_n; mu; transported previous impulse over s and its presence
+ * flag; overlap over the radius sum; plane flag; 0; 1) and 3 targets, the converged local
+ * impulse over s = |g| dt / w_n. */
+int scx_set_warm_model(scx_ctx* ctx, const float* params, uint32_t n);  /* n = SCX_MLP_PARAMS */
+int scx_get_warm_model(scx_ctx* ctx, float* params, uint32_t n);
+/* The last step's samples (that step must have used SCX_WARM_TRANSPORT or SCX_WARM_LEARNED):
+ * features 16 and targets 3 per contact, in active-set order. */
+int scx_get_samples(scx_ctx* ctx, uint32_t cap, float* features, float* targets, uint32_t* count);
+/* Mean-squared-error loss and its gradient (SCX_MLP_PARAMS, the layout above) over n
+ * samples with the current model, from the device's BF16 forward and backward kernels. */
+int scx_warm_model_gradients(scx_ctx* ctx, uint32_t n, const float* features, const float* targets, float* grads,
+                             double* loss);
+/* Adam training of the current model on the device: `epochs` passes over the n samples in
+ * shuffled batches of `batch` (seeded); loss = the final MSE on the first batch of samples. */
+int scx_train_warm_model(scx_ctx* ctx, uint32_t n, const float* features, const float* targets, uint32_t epochs,
+                         uint32_t batch, float lr, uint64_t seed, double* loss);
 
 /* UnconstrainedMap::flow on host arrays (q0, v0 -> q1, v1), gravity only. */
 int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const double* m, const double* g, double dt,

@@ -33,12 +33,18 @@ WARM_TRANSPORT = 3
 # Every symbol include/scx.h declares.
 EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_potential_overlaps", "scx_set_bodies",
            "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
-           "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing"]
+           "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
+           "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
+           "scx_train_warm_model"]
+
+MLP_PARAMS = 19075  # SCX_MLP_PARAMS
+WARM_LEARNED = 4
 
 _d = ctypes.c_double
 _pd = ctypes.POINTER(ctypes.c_double)
 _pu32 = ctypes.POINTER(ctypes.c_uint32)
 _pi32 = ctypes.POINTER(ctypes.c_int32)
+_pf = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 
 
@@ -102,6 +108,12 @@ def lib():
         L.scx_unconstrained_flow.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint32, _pd, _pd, _d, _pd, _pd, _pd, _pd]
         L.scx_set_timing.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_timing.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p), _pd, _pu32]
+        L.scx_set_warm_model.argtypes = [_vp, _pf, ctypes.c_uint32]
+        L.scx_get_warm_model.argtypes = [_vp, _pf, ctypes.c_uint32]
+        L.scx_get_samples.argtypes = [_vp, ctypes.c_uint32, _pf, _pf, _pu32]
+        L.scx_warm_model_gradients.argtypes = [_vp, ctypes.c_uint32, _pf, _pf, _pf, _pd]
+        L.scx_train_warm_model.argtypes = [_vp, ctypes.c_uint32, _pf, _pf, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_float, ctypes.c_uint64, _pd]
         _lib = L
     return _lib
 
@@ -269,3 +281,42 @@ class Context:
         ms = np.zeros(32)
         self._check(lib().scx_get_timing(self._h, 32, names, _p(ms), ctypes.byref(cnt)), "scx_get_timing")
         return [(names[i].decode(), float(ms[i])) for i in range(min(cnt.value, 32))]
+
+    # learned warm start
+    def set_warm_model(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._check(lib().scx_set_warm_model(self._h, p.ctypes.data_as(_pf), p.size), "scx_set_warm_model")
+
+    def warm_model(self):
+        p = np.empty(MLP_PARAMS, np.float32)
+        self._check(lib().scx_get_warm_model(self._h, p.ctypes.data_as(_pf), p.size), "scx_get_warm_model")
+        return p
+
+    def samples(self):
+        """(features [C][16], targets [C][3]) of the last step."""
+        cnt = ctypes.c_uint32(0)
+        lib().scx_get_samples(self._h, 0, None, None, ctypes.byref(cnt))
+        C = cnt.value
+        X = np.empty((C, 16), np.float32)
+        T = np.empty((C, 3), np.float32)
+        self._check(lib().scx_get_samples(self._h, C, X.ctypes.data_as(_pf), T.ctypes.data_as(_pf),
+                                          ctypes.byref(cnt)), "scx_get_samples")
+        return X, T
+
+    def warm_model_gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = ctypes.c_double(0.0)
+        self._check(lib().scx_warm_model_gradients(self._h, X.shape[0], X.ctypes.data_as(_pf), T.ctypes.data_as(_pf),
+                                                   g.ctypes.data_as(_pf), ctypes.byref(loss)),
+                    "scx_warm_model_gradients")
+        return g, loss.value
+
+    def train_warm_model(self, X, T, epochs, batch, lr, seed=1):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        self._check(lib().scx_train_warm_model(self._h, X.shape[0], X.ctypes.data_as(_pf), T.ctypes.data_as(_pf),
+                                               epochs, batch, lr, seed, ctypes.byref(loss)), "scx_train_warm_model")
+        return loss.value

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libscx.so")
 SCENES_LIB = os.path.join(HERE, "libscx_scenes.so")
 
-HIP_SOURCES = ["scx_prims.hip", "scx_broadphase.hip", "scx_contacts.hip", "scx_solver.hip", "scx_api.hip"]
+HIP_SOURCES = ["scx_prims.hip", "scx_broadphase.hip", "scx_contacts.hip", "scx_solver.hip", "scx_mlp.hip", "scx_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SCX_OFFLOAD_ARCH", "gfx950")
 

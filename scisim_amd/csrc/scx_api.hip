@@ -167,6 +167,10 @@ int alloc_contacts(scx_ctx* c, uint32_t C) {
   HIPC(dalloc(&d.cbeta, (size_t)(D - 1) * C));
   HIPC(dalloc(&d.crinit, (size_t)D * C));
   HIPC(dalloc(&d.cur_key, C));
+  HIPC(dalloc(&d.mlp_x, (size_t)16 * C));
+  HIPC(dalloc(&d.mlp_y, (size_t)3 * C));
+  HIPC(dalloc(&d.mlp_s, C));
+  d.features_valid = false;
   HIPC(dalloc(&d.adj, 2 * (size_t)C));
   HIPC(dalloc(&d.cstate, C));
   c->cap_perm = std::max<uint32_t>(C, c->cap_bodies);
@@ -209,6 +213,13 @@ void free_all(scx_ctx* c) {
   dfree(&d.tnar);
   dfree(&d.tr);
   if (d.pmask) hipFree(d.pmask);
+  float* mlpf[] = {d.mlp_params, d.mlp_grad, d.mlp_m, d.mlp_v, d.mlp_b, d.mlp_sse, d.mlp_x, d.mlp_y, d.mlp_s,
+                   d.mlp_pool_x, d.mlp_pool_t};
+  for (float* p : mlpf)
+    if (p) hipFree(p);
+  if (d.mlp_w) hipFree(d.mlp_w);
+  if (d.mlp_act) hipFree(d.mlp_act);
+  if (d.mlp_idx) hipFree(d.mlp_idx);
   if (d.prev_key) hipFree(d.prev_key);
   if (d.cur_key) hipFree(d.cur_key);
   if (d.grid) hipFree(d.grid);
@@ -251,6 +262,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   const int D = c->dim;
   if (c->events.empty()) timing_mark(c, "start");
   HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+  d.features_valid = false;
   if (!c->order_valid) {
     HIPC(launch_order_reset(c, n));
     c->order_valid = true;
@@ -315,8 +327,25 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   uint32_t max_colour = 0;
   if (C > 0) {
     HIPC(launch_contact_setup(c, nbb, npl, p->CoR, p->mu));
-    // warm start from the impulse cache (GeometricImpactFrictionMap.cpp:306)
-    if (p->warm_start != SCX_WARM_NONE) HIPC(launch_warm_start(c, C, p->warm_start));
+    // warm start from the impulse cache (GeometricImpactFrictionMap.cpp:306); the learned
+    // warm start reads the transported cache as an input and predicts the impulses
+    if (p->warm_start != SCX_WARM_NONE) {
+      const bool learned = p->warm_start == SCX_WARM_LEARNED;
+      HIPC(launch_warm_start(c, C, learned ? SCX_WARM_TRANSPORT : p->warm_start));
+      if (p->warm_start >= SCX_WARM_TRANSPORT) {
+        double g2 = 0.0;
+        for (int a = 0; a < D; ++a) g2 += p->g[a] * p->g[a];
+        const double vref = (g2 > 0.0 ? std::sqrt(g2) : 1.0) * p->dt;
+        HIPC(launch_contact_features(c, C, p->mu, vref));
+        d.features_valid = true;
+        if (learned) {
+          timing_mark(c, "features");
+          HIPC(launch_mlp_infer(c, C, d.mlp_x, d.mlp_y));
+          timing_mark(c, "mlp");
+          HIPC(launch_apply_prediction(c, C, p->mu));
+        }
+      }
+    }
     timing_mark(c, "setup");
     HIPC(launch_colouring(c, C));
     HIPC(launch_colour_sort(c, C));
@@ -342,7 +371,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
       HIPC(dalloc(&d.prev_r, (size_t)6 * want));  // kCacheWords per entry
       d.cap_cache = want;
     }
-    HIPC(launch_cache_store(c, C, p->warm_start));
+    HIPC(launch_cache_store(c, C, p->warm_start == SCX_WARM_LEARNED ? SCX_WARM_TRANSPORT : p->warm_start));
     d.prev_count = C;
   }
   timing_mark(c, "finish");
@@ -382,8 +411,10 @@ int check_params(scx_ctx* c, const scx_step_params* p) {
     return fail(c, SCX_E_ARG, "unknown integrator %d", p->integrator);
   if (p->precision != SCX_PRECISION_FP64 && p->precision != SCX_PRECISION_FP32_POLISH)
     return fail(c, SCX_E_ARG, "unknown precision mode %d", p->precision);
-  if (p->warm_start < SCX_WARM_NONE || p->warm_start > SCX_WARM_TRANSPORT)
+  if (p->warm_start < SCX_WARM_NONE || p->warm_start > SCX_WARM_LEARNED)
     return fail(c, SCX_E_ARG, "unknown warm start mode %d", p->warm_start);
+  if (p->warm_start == SCX_WARM_LEARNED && !c->d.mlp_ready)
+    return fail(c, SCX_E_STATE, "learned warm start needs a model (scx_set_warm_model or scx_train_warm_model)");
   // GeometricImpactFrictionMap.cpp:258-262: "Caching in 3 space not yet supported."
   if (p->warm_start == SCX_WARM_CACHE && c->dim == 3)
     return fail(c, SCX_E_ARG, "normal and friction impulse caching is 2D only (use SCX_WARM_NORMAL)");
@@ -859,6 +890,155 @@ int scx_potential_overlaps(scx_ctx* c, uint32_t n, const double* aabb_min, const
     return SCX_OK;
   }
   return fail(c, SCX_E_CAPACITY, "grid entry buffer kept overflowing");
+}
+
+// ---------------------------------------------------------------------------
+// Learned warm start: model I/O, training samples, training and gradients.
+// ---------------------------------------------------------------------------
+namespace {
+
+int mlp_alloc(scx_ctx* c) {
+  DeviceBuffers& d = c->d;
+  if (d.mlp_params) return SCX_OK;
+  HIPC(dalloc(&d.mlp_params, kMlpParams));
+  HIPC(dalloc(&d.mlp_grad, kMlpParams));
+  HIPC(dalloc(&d.mlp_m, kMlpParams));
+  HIPC(dalloc(&d.mlp_v, kMlpParams));
+  HIPC(dalloc(&d.mlp_w, (size_t)kMlpHid * kMlpIn + 2 * kMlpHid * kMlpHid + 2 * 16 * kMlpHid));
+  HIPC(dalloc(&d.mlp_b, (size_t)2 * kMlpHid + 16));
+  HIPC(dalloc(&d.mlp_sse, kMlpSseBlocks));
+  HIPC(hipMemsetAsync(d.mlp_m, 0, sizeof(float) * kMlpParams, c->stream));
+  HIPC(hipMemsetAsync(d.mlp_v, 0, sizeof(float) * kMlpParams, c->stream));
+  d.mlp_step = 0;
+  return SCX_OK;
+}
+
+// Device pool of n samples and activation space for batches of `batch`.
+int mlp_pool(scx_ctx* c, uint32_t n, uint32_t batch, const float* X, const float* T) {
+  DeviceBuffers& d = c->d;
+  if (n > d.mlp_pool_cap) {
+    HIPC(dalloc(&d.mlp_pool_x, (size_t)16 * n));
+    HIPC(dalloc(&d.mlp_pool_t, (size_t)3 * n));
+    HIPC(dalloc(&d.mlp_idx, n));
+    d.mlp_pool_cap = n;
+  }
+  const uint32_t tiles = (batch + 15) / 16;
+  if (tiles > d.mlp_cap_tiles) {
+    // H1T, H2T, G1T, G2T: 128 x 16 each per tile; GY, XT: 16 x 16 each
+    HIPC(dalloc(&d.mlp_act, (size_t)tiles * (4 * kMlpHid * 16 + 2 * 256)));
+    d.mlp_cap_tiles = tiles;
+  }
+  HIPC(hipMemcpyAsync(d.mlp_pool_x, X, sizeof(float) * 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(d.mlp_pool_t, T, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  return SCX_OK;
+}
+
+// Sum of the squared-error partials of the last gradient pass.
+int mlp_sse(scx_ctx* c, double* out) {
+  std::vector<float> part(kMlpSseBlocks);
+  HIPC(hipMemcpyAsync(part.data(), c->d.mlp_sse, sizeof(float) * kMlpSseBlocks, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  double s = 0.0;
+  for (float x : part) s += x;
+  *out = s;
+  return SCX_OK;
+}
+
+}  // namespace
+
+int scx_set_warm_model(scx_ctx* c, const float* params, uint32_t n) {
+  if (!c) return SCX_E_ARG;
+  if (!params || n != (uint32_t)kMlpParams) return fail(c, SCX_E_ARG, "expected %d parameters", kMlpParams);
+  HIPC(hipSetDevice(c->device));
+  if (mlp_alloc(c) != SCX_OK) return SCX_E_HIP;
+  HIPC(hipMemcpyAsync(c->d.mlp_params, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemsetAsync(c->d.mlp_m, 0, sizeof(float) * kMlpParams, c->stream));
+  HIPC(hipMemsetAsync(c->d.mlp_v, 0, sizeof(float) * kMlpParams, c->stream));
+  c->d.mlp_step = 0;
+  HIPC(launch_mlp_pack(c));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->d.mlp_ready = true;
+  return SCX_OK;
+}
+
+int scx_get_warm_model(scx_ctx* c, float* params, uint32_t n) {
+  if (!c) return SCX_E_ARG;
+  if (!params || n != (uint32_t)kMlpParams) return fail(c, SCX_E_ARG, "expected %d parameters", kMlpParams);
+  if (!c->d.mlp_ready) return fail(c, SCX_E_STATE, "no warm-start model");
+  HIPC(hipSetDevice(c->device));
+  HIPC(hipMemcpy(params, c->d.mlp_params, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return SCX_OK;
+}
+
+int scx_get_samples(scx_ctx* c, uint32_t cap, float* features, float* targets, uint32_t* count) {
+  if (!c) return SCX_E_ARG;
+  if (count) *count = 0;
+  if (!c->d.features_valid) return fail(c, SCX_E_STATE, "the last step computed no features (warm start 3 or 4)");
+  HIPC(hipSetDevice(c->device));
+  const uint32_t C = c->last_ncontacts;
+  if (count) *count = C;
+  if (C > cap) return fail(c, SCX_E_CAPACITY, "need room for %u samples", C);
+  if (C == 0) return SCX_OK;
+  HIPC(launch_sample_targets(c, C));
+  if (features)
+    HIPC(hipMemcpyAsync(features, c->d.mlp_x, sizeof(float) * 16 * (size_t)C, hipMemcpyDeviceToHost, c->stream));
+  if (targets)
+    HIPC(hipMemcpyAsync(targets, c->d.mlp_y, sizeof(float) * 3 * (size_t)C, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return SCX_OK;
+}
+
+int scx_warm_model_gradients(scx_ctx* c, uint32_t n, const float* features, const float* targets, float* grads,
+                             double* loss) {
+  if (!c) return SCX_E_ARG;
+  if (n == 0 || !features || !targets) return fail(c, SCX_E_ARG, "empty batch");
+  if (!c->d.mlp_ready) return fail(c, SCX_E_STATE, "no warm-start model");
+  HIPC(hipSetDevice(c->device));
+  if (mlp_pool(c, n, n, features, targets) != SCX_OK) return SCX_E_HIP;
+  HIPC(launch_mlp_gradients(c, n, nullptr, 0, c->d.mlp_pool_x, c->d.mlp_pool_t));
+  if (grads) HIPC(hipMemcpyAsync(grads, c->d.mlp_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c->stream));
+  double sse = 0.0;
+  if (mlp_sse(c, &sse) != SCX_OK) return SCX_E_HIP;
+  if (loss) *loss = sse / ((double)n * kMlpOut);
+  return SCX_OK;
+}
+
+int scx_train_warm_model(scx_ctx* c, uint32_t n, const float* features, const float* targets, uint32_t epochs,
+                         uint32_t batch, float lr, uint64_t seed, double* loss) {
+  if (!c) return SCX_E_ARG;
+  if (n == 0 || !features || !targets || batch == 0) return fail(c, SCX_E_ARG, "empty training set or batch");
+  if (!c->d.mlp_ready) return fail(c, SCX_E_STATE, "no warm-start model to train (scx_set_warm_model first)");
+  HIPC(hipSetDevice(c->device));
+  batch = std::min(batch, n);
+  if (mlp_pool(c, n, batch, features, targets) != SCX_OK) return SCX_E_HIP;
+  std::vector<uint32_t> idx(n);
+  uint64_t state = seed;
+  for (uint32_t e = 0; e < epochs; ++e) {
+    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+    for (uint32_t i = n; i > 1; --i) {  // Fisher-Yates, splitmix64
+      uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      std::swap(idx[i - 1], idx[z % i]);
+    }
+    HIPC(hipMemcpyAsync(c->d.mlp_idx, idx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    for (uint32_t b0 = 0; b0 + batch <= n; b0 += batch) {
+      HIPC(launch_mlp_gradients(c, batch, c->d.mlp_idx, b0, c->d.mlp_pool_x, c->d.mlp_pool_t));
+      HIPC(launch_mlp_adam(c, lr, ++c->d.mlp_step));
+      HIPC(launch_mlp_pack(c));
+    }
+    // idx is reused by the next epoch's upload only after these launches (same stream)
+    HIPC(hipStreamSynchronize(c->stream));
+  }
+  if (loss) {
+    // mean squared error over (up to) the first batch-sized slice of the pool
+    HIPC(launch_mlp_gradients(c, batch, nullptr, 0, c->d.mlp_pool_x, c->d.mlp_pool_t));
+    double sse = 0.0;
+    if (mlp_sse(c, &sse) != SCX_OK) return SCX_E_HIP;
+    *loss = sse / ((double)batch * kMlpOut);
+  }
+  return SCX_OK;
 }
 
 int scx_unconstrained_flow(scx_ctx* c, int32_t integrator, uint32_t n, const double* m, const double* g, double dt,

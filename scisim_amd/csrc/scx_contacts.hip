@@ -601,6 +601,156 @@ __global__ void __launch_bounds__(kBlock) k_cache_store(uint32_t C, int dim, int
   }
 }
 
+// Per-contact inputs of the learned warm start (16 FP32, reference order), with the
+// impulse scale s = vref / w_n (the impulse that stops a relative velocity vref = |g| dt):
+//   0-2 normal n (world; 2D z = 0)
+//   3-5 pre-solve relative velocity in the contact frame, u = H v0 + w, over vref
+//   6   w_t / w_n (tangential over normal Delassus diagonal)
+//   7   mu
+//   8-10 the transported previous impulse (local frame) over s, 11: 1 if it is non-zero
+//   12  overlap at q0 over the radius sum (plane: over the radius)
+//   13  1 for a plane contact, 14: 0, 15: 1
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_contact_features(uint32_t C, const uint32_t* __restrict__ ca,
+                                                             const uint32_t* __restrict__ cb,
+                                                             const int32_t* __restrict__ cplane,
+                                                             const double* __restrict__ basis,
+                                                             const double* __restrict__ la,
+                                                             const double* __restrict__ lb,
+                                                             const double* __restrict__ wn,
+                                                             const double* __restrict__ wt, double mu,
+                                                             const double* __restrict__ forcing,
+                                                             const double* __restrict__ q0,
+                                                             const double* __restrict__ v0,
+                                                             const double* __restrict__ w0,
+                                                             const double* __restrict__ r,
+                                                             const double* __restrict__ px,
+                                                             const double* __restrict__ rprev, double vref,
+                                                             float* __restrict__ X, float* __restrict__ S) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t a = ca[k], b = cb[k];
+  const bool hasb = b != kNoBody;
+  const double* B = basis + (size_t)D * D * k;
+  const double s = vref / wn[k];
+  float f[16];
+  for (int i = 0; i < 16; ++i) f[i] = 0.f;
+  for (int i = 0; i < D; ++i) f[i] = (float)B[i];
+  // u = H v0 + w: dv projected on the frame, minus the angular terms (arms -la n, +lb n)
+  double dv[3] = {0, 0, 0};
+  for (int i = 0; i < D; ++i) dv[i] = v0[(size_t)D * a + i] - (hasb ? v0[(size_t)D * b + i] : 0.0);
+  if (D == 3) {
+    const double* n = B;
+    const double* wa = w0 + 3 * (size_t)a;
+    double ca_[3] = {wa[1] * n[2] - wa[2] * n[1], wa[2] * n[0] - wa[0] * n[2], wa[0] * n[1] - wa[1] * n[0]};
+    double cb_[3] = {0, 0, 0};
+    if (hasb) {
+      const double* wb = w0 + 3 * (size_t)b;
+      cb_[0] = wb[1] * n[2] - wb[2] * n[1];
+      cb_[1] = wb[2] * n[0] - wb[0] * n[2];
+      cb_[2] = wb[0] * n[1] - wb[1] * n[0];
+    }
+    for (int i = 0; i < 3; ++i) dv[i] = dv[i] - la[k] * ca_[i] - lb[k] * cb_[i];
+  }
+  for (int j = 0; j < D; ++j) {
+    double u = 0.0;
+    for (int i = 0; i < D; ++i) u += B[j * D + i] * dv[i];
+    if (j == 0) u += forcing[k];
+    f[3 + j] = (float)(u / vref);
+  }
+  f[6] = (float)(wt[k] / wn[k]);
+  f[7] = (float)mu;
+  bool prev = false;
+  for (int j = 0; j < D; ++j) {
+    const double rp = rprev[(size_t)D * k + j];
+    f[8 + j] = (float)(rp / s);
+    prev = prev || rp != 0.0;
+  }
+  f[11] = prev ? 1.f : 0.f;
+  if (hasb) {
+    double d2 = 0.0;
+    for (int i = 0; i < D; ++i) {
+      const double dx = q0[(size_t)D * a + i] - q0[(size_t)D * b + i];
+      d2 += dx * dx;
+    }
+    const double rs = r[a] + r[b];
+    f[12] = (float)((rs - sqrt(d2)) / rs);
+  } else {
+    const int p = cplane[k];
+    double h = 0.0;
+    for (int i = 0; i < D; ++i) h += B[i] * (q0[(size_t)D * a + i] - px[(size_t)D * p + i]);
+    f[12] = (float)((r[a] - h) / r[a]);
+    f[13] = 1.f;
+  }
+  f[15] = 1.f;
+  float4* xo = reinterpret_cast<float4*>(X + (size_t)16 * k);
+  for (int i = 0; i < 4; ++i) xo[i] = make_float4(f[4 * i], f[4 * i + 1], f[4 * i + 2], f[4 * i + 3]);
+  S[k] = (float)s;
+}
+
+// Predicted impulses (over s) into the warm start: r = s y, normal clamped at zero and
+// the tangential part projected onto the friction cone |r_T| <= mu r_n.
+__global__ void __launch_bounds__(kBlock) k_apply_prediction(uint32_t C, int dim, double mu,
+                                                             const float* __restrict__ Y,
+                                                             const float* __restrict__ S,
+                                                             double* __restrict__ rinit) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const double s = S[k];
+  const double rn = fmax(0.0, s * (double)Y[(size_t)3 * k]);
+  double rt[2] = {0.0, 0.0}, nt = 0.0;
+  for (int j = 1; j < dim; ++j) {
+    rt[j - 1] = s * (double)Y[(size_t)3 * k + j];
+    nt += rt[j - 1] * rt[j - 1];
+  }
+  nt = sqrt(nt);
+  const double lim = mu * rn;
+  const double sc = (nt > lim && nt > 0.0) ? lim / nt : 1.0;
+  rinit[(size_t)dim * k] = rn;
+  for (int j = 1; j < dim; ++j) rinit[(size_t)dim * k + j] = rt[j - 1] * sc;
+}
+
+// Training targets of the last solve: the converged local impulses over s.
+__global__ void __launch_bounds__(kBlock) k_sample_targets(uint32_t C, int dim, const double* __restrict__ alpha,
+                                                           const double* __restrict__ beta,
+                                                           const float* __restrict__ S, float* __restrict__ T) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const double s = S[k];
+  T[(size_t)3 * k] = (float)(alpha[k] / s);
+  T[(size_t)3 * k + 1] = (dim > 1) ? (float)(beta[(size_t)(dim - 1) * k] / s) : 0.f;
+  T[(size_t)3 * k + 2] = (dim > 2) ? (float)(beta[(size_t)(dim - 1) * k + 1] / s) : 0.f;
+}
+
+hipError_t launch_contact_features(scx_ctx* c, uint32_t C, double mu, double vref) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  DeviceBuffers& d = c->d;
+  if (c->dim == 2)
+    k_contact_features<2><<<nblk, kBlock, 0, c->stream>>>(C, d.ca, d.cb, d.cplane, d.basis, d.cla, d.clb, d.cwn,
+                                                           d.cwt, mu, d.cforcing, d.q0, d.v0, d.w0, d.r, d.px,
+                                                           d.crinit, vref, d.mlp_x, d.mlp_s);
+  else
+    k_contact_features<3><<<nblk, kBlock, 0, c->stream>>>(C, d.ca, d.cb, d.cplane, d.basis, d.cla, d.clb, d.cwn,
+                                                           d.cwt, mu, d.cforcing, d.q0, d.v0, d.w0, d.r, d.px,
+                                                           d.crinit, vref, d.mlp_x, d.mlp_s);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_prediction(scx_ctx* c, uint32_t C, double mu) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  k_apply_prediction<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mu, c->d.mlp_y, c->d.mlp_s, c->d.crinit);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_targets(scx_ctx* c, uint32_t C) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0) return hipSuccess;
+  k_sample_targets<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, c->d.calpha, c->d.cbeta, c->d.mlp_s, c->d.mlp_y);
+  return hipGetLastError();
+}
+
 hipError_t launch_warm_start(scx_ctx* c, uint32_t C, int mode) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;

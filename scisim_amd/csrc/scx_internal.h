@@ -18,6 +18,18 @@ constexpr int kScanTile = kBlock * kScanItems;
 constexpr uint32_t kNoBody = 0xFFFFFFFFu;
 constexpr int kMaxColours = 64;
 
+// Learned warm start: 16-128-128-3 perceptron; FP32 parameter vector layout
+// W1 [16][128], b1 [128], W2 [128][128], b2 [128], W3 [128][3], b3 [3].
+constexpr int kMlpIn = 16, kMlpHid = 128, kMlpOut = 3;
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpIn * kMlpHid;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHid;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHid * kMlpHid;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHid;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHid * kMlpOut;
+constexpr int kMlpParams = kMlpOffB3 + kMlpOut;
+constexpr int kMlpSseBlocks = 1024;   // most workgroups of a training pass (loss partials)
+
 // Grid parameters, computed on the device each step (reference formula of
 // ball2d/SpatialGridDetector.cpp:76-104: min corner - 2e-6, h = max axis of the summed
 // extents / n). Device-resident so that no host round trip is needed.
@@ -145,6 +157,26 @@ struct DeviceBuffers {
   uint32_t prev_count = 0;
   uint32_t cap_cache = 0;
   unsigned long long* cur_key = nullptr;   // this step's keys (cap_contacts)
+  // learned warm start
+  float* mlp_params = nullptr;   // kMlpParams FP32 master
+  float* mlp_grad = nullptr;     // kMlpParams
+  float* mlp_m = nullptr;        // Adam moments
+  float* mlp_v = nullptr;
+  uint16_t* mlp_w = nullptr;     // packed BF16 weights (k_mlp_pack layouts)
+  float* mlp_b = nullptr;        // packed biases
+  uint16_t* mlp_act = nullptr;   // training activations/gradients per sample tile
+  uint32_t mlp_cap_tiles = 0;    // sample tiles mlp_act holds
+  float* mlp_sse = nullptr;      // kMlpSseBlocks loss partials
+  bool mlp_ready = false;        // weights set
+  uint32_t mlp_step = 0;         // Adam steps taken
+  float* mlp_x = nullptr;        // per contact: 16 features (cap_contacts)
+  float* mlp_y = nullptr;        // per contact: 3 predictions / targets
+  float* mlp_s = nullptr;        // per contact: impulse scale
+  float* mlp_pool_x = nullptr;   // training pool (cap mlp_pool_cap)
+  float* mlp_pool_t = nullptr;
+  uint32_t* mlp_idx = nullptr;
+  uint32_t mlp_pool_cap = 0;
+  bool features_valid = false;   // the last step computed mlp_x / mlp_s
   StepCounters* ctr = nullptr;
   uint32_t* barrier = nullptr;       // grid barrier state of the persistent kernels (zeroed once)
   StepCounters* ctr_host = nullptr;  // pinned
@@ -209,6 +241,18 @@ hipError_t launch_impulses(scx_ctx* c, uint32_t C);
 // step's cache into d.crinit; then this step's alpha/beta into the cache.
 hipError_t launch_warm_start(scx_ctx* c, uint32_t C, int mode);
 hipError_t launch_cache_store(scx_ctx* c, uint32_t C, int mode);
+// Learned warm start: per-contact features and impulse scales (d.mlp_x, d.mlp_s) from
+// the transported cache in d.crinit; predictions d.mlp_y into d.crinit; the converged
+// impulses over the scales (training targets) into d.mlp_y.
+hipError_t launch_contact_features(scx_ctx* c, uint32_t C, double mu, double vref);
+hipError_t launch_apply_prediction(scx_ctx* c, uint32_t C, double mu);
+hipError_t launch_sample_targets(scx_ctx* c, uint32_t C);
+// scx_mlp.hip
+hipError_t launch_mlp_pack(scx_ctx* c);
+hipError_t launch_mlp_infer(scx_ctx* c, uint32_t n, const float* X, float* Y);
+hipError_t launch_mlp_gradients(scx_ctx* c, uint32_t nb, const uint32_t* idx, uint32_t b0, const float* X,
+                                const float* Tgt);
+hipError_t launch_mlp_adam(scx_ctx* c, float lr, uint32_t step);
 int solver_max_blocks(scx_ctx* c);
 int colour_max_blocks(scx_ctx* c);
 }  // namespace scx

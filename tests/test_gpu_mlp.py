@@ -1,0 +1,67 @@
+"""The learned warm start's perceptron on the device: BF16 MFMA forward/backward
+gradients against the CPU restatement (oracle/mlp.py, BF16 rounding emulated), and Adam
+training on the device."""
+
+import numpy as np
+import pytest
+
+import scisim_amd as scx
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = scx.Context(3, 1000, max_planes=8)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n", [16, 200, 1000])
+def test_device_gradients_match_oracle(ctx, n):
+    rng = np.random.default_rng(n)
+    p = mlp.init(seed=7)
+    p[mlp.NPARAMS - 3:] = np.float32(0.05)
+    ctx.set_warm_model(p)
+    assert np.array_equal(ctx.warm_model(), p)
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    T = rng.normal(size=(n, 3)).astype(np.float32)
+    g_dev, loss_dev = ctx.warm_model_gradients(X, T)
+    loss_ref, g_ref = mlp.loss_and_grad(p, X, T, emulate_bf16=True)
+    loss_ex, g_ex = mlp.loss_and_grad(p, X, T)
+    _say("n", n, "loss dev", loss_dev, "oracle bf16", loss_ref, "exact", loss_ex)
+    assert abs(loss_dev - loss_ref) <= 2e-3 * loss_ref
+    k = 0
+    for name, shape in mlp.SHAPES:
+        m = int(np.prod(shape))
+        a, b = g_dev[k:k + m].astype(np.float64), g_ref[k:k + m]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        _say(name, "relative gradient error", err)
+        assert err <= 2e-2, name
+        k += m
+    # and the device gradient points the way the exact one does
+    assert g_dev.astype(np.float64) @ g_ex >= 0.98 * np.linalg.norm(g_dev) * np.linalg.norm(g_ex)
+
+
+def test_device_training_fits_a_smooth_map(ctx):
+    """Adam on the device drives the loss of a learnable map well below its start."""
+    rng = np.random.default_rng(11)
+    n = 8192
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    A = rng.normal(size=(16, 3)) / 4.0
+    T = (np.tanh(X @ A) + 0.1 * (X[:, :3] ** 2)).astype(np.float32)
+    p = mlp.init(seed=3)
+    ctx.set_warm_model(p)
+    _, loss0 = ctx.warm_model_gradients(X[:1024], T[:1024])
+    loss = ctx.train_warm_model(X, T, epochs=20, batch=512, lr=2e-3, seed=5)
+    _say("loss before", loss0, "after", loss)
+    assert np.isfinite(loss) and loss < 0.1 * loss0
+    # the trained parameters are what the device reports, and the oracle agrees on the loss
+    q = ctx.warm_model()
+    l_ref, _ = mlp.loss_and_grad(q, X[:512], T[:512], emulate_bf16=True)
+    assert abs(l_ref - loss) <= 5e-3 * max(l_ref, 1e-12) + 1e-6
