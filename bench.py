@@ -7,7 +7,11 @@ Workloads (synthetic, seeded):
         residual tolerance 1e-6.
   cfg2: rigidbody3d, 262,144 spheres, log-normal radii (median 0.05, sigma 0.25),
         poured into a box with 5 planes (sequential deposition into a resting pile,
-        then untimed settling steps), mu = 0.4, CoR = 0, Verlet, dt = 0.005.
+        then untimed settling steps), mu = 0.4, CoR = 0, Verlet, dt = 0.005; plain
+        coloured Gauss-Seidel (cold start) unless --warm says otherwise.
+  cfg3: cfg2 with the learned warm start: samples of `--collect` steps, the perceptron
+        trained on them on the device (untimed), then the timed steps with it; the cold
+        and transported-cache warm starts are measured from the same state for comparison.
 
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
 narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final
@@ -42,10 +46,51 @@ def make_scene(cfg, seed):
         s = scenes.spheres_pile(n=262144, rmed=0.05, sigma=0.25, seed=seed)
         prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1,
                    integrator=1)
-        desc = {"workload": "spheres3d_pile_262k", "bodies": 262144, "dim": 3, "radii": "lognormal(0.05,0.25)",
+        desc = {"workload": "spheres3d_pile_262k" + ("_learned_warm_start" if cfg == "cfg3" else ""),
+                "bodies": 262144, "dim": 3, "radii": "lognormal(0.05,0.25)",
                 "packing": "poured pile (sequential deposition), settled", "planes": 5, "mu": 0.4, "CoR": 0.0,
                 "dt": 0.005, "integrator": "verlet", "tol": 1e-6}
     return s, prm, desc
+
+
+def train_learned_warm_start(ctx, s, prm, precision, args, log):
+    """cfg3: samples of `collect` steps under the transported cache (a quarter of the
+    contacts of each step, seeded), the perceptron trained on them on the device, then the
+    sweeps and step times of the cold, transported and learned warm starts over the same
+    `steps` steps from the same state. The context ends in that state with the model set."""
+    import scisim_amd as scx
+    from scisim_amd import warmstart
+    rng = np.random.default_rng(args.seed)
+    tp = scx.step_params(precision=precision, **dict(prm, warm_start=scx.WARM_TRANSPORT))
+    Xs, Ts = [], []
+    for k in range(args.collect):
+        ctx.step(tp)
+        X, T = ctx.samples()
+        keep = rng.random(X.shape[0]) < 0.25
+        Xs.append(X[keep])
+        Ts.append(T[keep])
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    ctx.set_warm_model(warmstart.init_params(seed=args.seed))
+    t0 = time.perf_counter()
+    loss = ctx.train_warm_model(X, T, epochs=12, batch=2048, lr=1e-3, seed=args.seed)
+    train_s = time.perf_counter() - t0
+    log("trained on", X.shape[0], "samples, loss", loss, "%.1fs" % train_s)
+    q, v, w = ctx.get_state()
+    out = {"training_samples": int(X.shape[0]), "collect_steps": args.collect, "epochs": 12, "train_s": train_s,
+           "final_mse": loss}
+    for name, mode in (("none", scx.WARM_NONE), ("transport", scx.WARM_TRANSPORT), ("learned", scx.WARM_LEARNED)):
+        ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
+        ctx.step(tp)  # the same cache for every mode
+        sw = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sw.append(ctx.step(scx.step_params(precision=precision, **dict(prm, warm_start=mode)))["sweeps"])
+        dt = time.perf_counter() - t0
+        out[name] = {"mean_sweeps": float(np.mean(sw)), "body_steps_per_s": s["r"].size * args.steps / dt}
+        log(name, out[name])
+    ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
+    ctx.step(tp)
+    return out
 
 
 def cpu_baseline(s, prm, budget_s, max_steps):
@@ -54,7 +99,7 @@ def cpu_baseline(s, prm, budget_s, max_steps):
     t0 = time.perf_counter()
     k = 0
     while k < max_steps and (time.perf_counter() - t0) < budget_s:
-        sim.step(order=0, **prm)
+        sim.step(order=0, **dict(prm, warm_start=0))  # the reference's cold So-bogus solve
         k += 1
     dt = time.perf_counter() - t0
     n = s["r"].size
@@ -67,9 +112,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2"])
+    ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2", "cfg3"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--settle", type=int, default=20, help="untimed settling steps (cfg2)")
+    ap.add_argument("--settle", type=int, default=300, help="untimed settling steps of the pile (cfg2, cfg3)")
+    ap.add_argument("--collect", type=int, default=40, help="steps whose samples train the warm start (cfg3)")
     ap.add_argument("--warm", default="none", choices=["none", "cache", "normal", "transport"],
                     help="impulse cache warm start (ImpulsesToCache)")
     ap.add_argument("--seed", type=int, default=1)
@@ -78,11 +124,23 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 plain device copies of BYTES (the reference kernel that scales the HBM "
+                         "byte counters of rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (requested --gpus %d); multi-GPU scaling is not part of "
               "it" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        import torch
+        src = torch.empty(args.copy_calibration // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "copies": 5}))
+        return
 
     import scisim_amd as scx
     from scisim_amd import roofline as rl
@@ -100,18 +158,27 @@ def main():
     ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s.get("inertia"))
     precision = scx.FP32_POLISH if args.precision == "fp32" else scx.FP64
     it = 1
-    if args.config == "cfg2" and args.settle > 0:
-        sp = scx.step_params(precision=precision, **dict(prm, max_iters=500))
+    pile = args.config in ("cfg2", "cfg3")
+    if pile and args.settle > 0:
+        # the deposited pile settles under the transported impulse cache (untimed)
+        sp = scx.step_params(precision=precision, **dict(prm, max_iters=1000, warm_start=scx.WARM_TRANSPORT))
         t0 = time.perf_counter()
-        ctx.run(sp, args.settle)
-        log("settled %d steps in %.1fs" % (args.settle, time.perf_counter() - t0))
+        for k in range(args.settle):
+            st = ctx.step(sp)
+            if k % 25 == 24:
+                log("settle", k, "contacts", st["ncontacts"], "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
         it += args.settle
+        desc["settle_steps"] = args.settle
     if args.max_iters is not None:
         prm["max_iters"] = args.max_iters
     if args.eval_every is not None:
         prm["eval_every"] = args.eval_every
+    comparison = None
+    if args.config == "cfg3":
+        comparison = train_learned_warm_start(ctx, s, prm, precision, args, log)
+        args.warm = "learned"
     prm["warm_start"] = {"none": scx.WARM_NONE, "cache": scx.WARM_CACHE, "normal": scx.WARM_NORMAL,
-                         "transport": scx.WARM_TRANSPORT}[args.warm]
+                         "transport": scx.WARM_TRANSPORT, "learned": scx.WARM_LEARNED}[args.warm]
     desc["warm_start"] = args.warm
     params = scx.step_params(precision=precision, iteration=it, **prm)
     for _ in range(args.warmup):
@@ -165,7 +232,8 @@ def main():
         "dtype": "f32 sweeps + f64 residual/polish" if fp32 else "f64",
         "data": "synthetic",
         "config": dict(desc, mean_contacts=float(np.mean(contacts)), mean_sweeps=float(np.mean(sweeps)),
-                       phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()}),
+                       phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()},
+                       **({"warm_start_comparison": comparison} if comparison else {})),
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": rl.HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / rl.HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": kbytes / (launches * args.steps),

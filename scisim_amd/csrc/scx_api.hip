@@ -90,6 +90,8 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
   HIPC(dalloc(&d.bv64, 8 * (size_t)N));
   HIPC(dalloc(&d.bv32, 8 * (size_t)N));
   HIPC(dalloc(&d.slot_of, N));
+  HIPC(dalloc(&d.lmap, N));
+  HIPC(hipMemset(d.lmap, 0xFF, sizeof(uint32_t) * N));
   d.sid = nullptr;
   d.slot_map = nullptr;
   HIPC(dalloc(&d.aabb_min, (size_t)D * N));
@@ -201,6 +203,7 @@ void free_all(scx_ctx* c) {
     if (p) hipFree(p);
   if (d.bv32) hipFree(d.bv32);
   dfree(&d.slot_of);
+  dfree(&d.lmap);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -935,8 +938,9 @@ int mlp_pool(scx_ctx* c, uint32_t n, uint32_t batch, const float* X, const float
 
 // Sum of the squared-error partials of the last gradient pass.
 int mlp_sse(scx_ctx* c, double* out) {
-  std::vector<float> part(kMlpSseBlocks);
-  HIPC(hipMemcpyAsync(part.data(), c->d.mlp_sse, sizeof(float) * kMlpSseBlocks, hipMemcpyDeviceToHost, c->stream));
+  const uint32_t nb = std::max<uint32_t>(1u, c->d.mlp_sse_blocks);
+  std::vector<float> part(nb);
+  HIPC(hipMemcpyAsync(part.data(), c->d.mlp_sse, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   double s = 0.0;
   for (float x : part) s += x;

@@ -98,6 +98,7 @@ struct DeviceBuffers {
   // broad phase's home-cell order (spatially sorted), so that the bodies of neighbouring
   // contacts sit in neighbouring memory; scx_friction_solve uses the identity.
   uint32_t* slot_of = nullptr;  // N
+  uint32_t* lmap = nullptr;     // N, all ones between solves (endpoint owners of the LDS solve)
   const uint32_t* sid = nullptr;        // slot -> id of the current solve, nullptr = identity
   const uint32_t* slot_map = nullptr;   // id -> slot of the current solve, nullptr = identity
   // planes
@@ -167,6 +168,7 @@ struct DeviceBuffers {
   uint16_t* mlp_act = nullptr;   // training activations/gradients per sample tile
   uint32_t mlp_cap_tiles = 0;    // sample tiles mlp_act holds
   float* mlp_sse = nullptr;      // kMlpSseBlocks loss partials
+  uint32_t mlp_sse_blocks = 0;   // partials written by the last gradient pass
   bool mlp_ready = false;        // weights set
   uint32_t mlp_step = 0;         // Adam steps taken
   float* mlp_x = nullptr;        // per contact: 16 features (cap_contacts)

@@ -359,6 +359,7 @@ hipError_t launch_mlp_gradients(scx_ctx* c, uint32_t nb, const uint32_t* idx, ui
   TrainBufs B{d.mlp_act, d.mlp_act + act, d.mlp_act + 2 * act, d.mlp_act + 3 * act, d.mlp_act + 4 * act,
               d.mlp_act + 4 * act + (size_t)d.mlp_cap_tiles * 256, d.mlp_sse};
   const uint32_t blocks = std::min<uint32_t>((ntiles + 3) / 4, (uint32_t)kMlpSseBlocks);
+  d.mlp_sse_blocks = blocks;  // the loss partials this pass writes
   k_mlp_fwdbwd<<<blocks, kBlock, 0, c->stream>>>(nb, idx, b0, X, Tgt, d.mlp_w, d.mlp_b,
                                                   1.0f / ((float)nb * (float)kMlpOut), B);
   k_mlp_wgrad<<<20, kBlock, 0, c->stream>>>(ntiles, B, d.mlp_grad);

@@ -182,6 +182,7 @@ struct GSArgs {
   double tol;
   int phase;                // 0: only phase, 1: FP32 phase, 2: FP64 polish after FP32
   int apply_init;           // reset velocities to v0 and apply the records' impulses
+  uint32_t* lmap;           // per body slot: owning endpoint in the LDS solve (all ones when idle)
 };
 
 template <int D, typename Real>
@@ -404,6 +405,284 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same solve for a contact set that one workgroup holds in LDS: the records, the
+// impulses and the velocity records of every touched body live in LDS for the whole
+// solve, so a colour pass is LDS traffic and a workgroup barrier instead of dependent
+// global-memory round trips. Bodies are addressed by endpoint: each touched body is
+// owned by the smallest endpoint 2k + q (contact k, side q) that references it (an
+// atomicMin over the per-slot map lmap, restored to all ones afterwards).
+// ---------------------------------------------------------------------------
+template <int D, typename Real>
+struct LdsLayout {
+  using V = typename Vec4<Real>::type;
+  // bytes per contact: 2 endpoints x body record, ids, frame, scalars, impulse
+  static constexpr size_t kBodyBytes = 2 * kBodyVecs<D> * sizeof(V);
+  static constexpr size_t kRecBytes = 2 * sizeof(uint32_t) + (D == 3 ? 8 : 2) * sizeof(Real) + 4 * sizeof(Real) +
+                                      3 * sizeof(Real);
+  static size_t bytes(uint32_t C) { return (size_t)C * (kBodyBytes + kRecBytes) + 64; }
+};
+
+template <int D, typename Real>
+__global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
+  using V = typename Vec4<Real>::type;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+  const uint32_t C = A.C;
+  V* sb = reinterpret_cast<V*>(lds);                                       // 2C x kBodyVecs
+  uint32_t* ea = reinterpret_cast<uint32_t*>(sb + (size_t)2 * C * kBodyVecs<D>);
+  uint32_t* eb = ea + C;
+  Real* sn = reinterpret_cast<Real*>(eb + C);                              // D == 3: n(3) s(3) la lb; 2D: n(2)
+  Real* ss = sn + (size_t)(D == 3 ? 3 : 2) * C;
+  Real* sl = (D == 3) ? ss + (size_t)3 * C : ss;                          // la, lb (3D)
+  Real* sw = (D == 3) ? sl + (size_t)2 * C : ss;                          // wn, wt, mu, f
+  Real* sr = sw + (size_t)4 * C;                                           // impulses, 3 per contact
+  __shared__ uint32_t off[kMaxColours + 1];
+  __shared__ double s_red[kGSBlock / 64];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t ncol = A.ctr->ncolours;
+  for (uint32_t c = tid; c <= ncol; c += kGSBlock) off[c] = A.coloff[c];
+  // endpoint owners
+  for (uint32_t k = tid; k < C; k += kGSBlock) {
+    atomicMin(&A.lmap[A.c.a[k]], 2 * k);
+    if (A.c.b[k] != kNoBody) atomicMin(&A.lmap[A.c.b[k]], 2 * k + 1);
+  }
+  __syncthreads();
+  const V* gb = reinterpret_cast<const V*>(A.body);
+  for (uint32_t k = tid; k < C; k += kGSBlock) {
+    const uint32_t a = A.c.a[k], b = A.c.b[k];
+    const uint32_t oa = __hip_atomic_load(&A.lmap[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t ob = (b != kNoBody) ? __hip_atomic_load(&A.lmap[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                       : kNoBody;
+    ea[k] = oa;
+    eb[k] = ob;
+    if (oa == 2 * k)
+      for (int h = 0; h < kBodyVecs<D>; ++h) sb[(size_t)kBodyVecs<D> * oa + h] = gb[(size_t)kBodyVecs<D> * a + h];
+    if (ob == 2 * k + 1)
+      for (int h = 0; h < kBodyVecs<D>; ++h) sb[(size_t)kBodyVecs<D> * ob + h] = gb[(size_t)kBodyVecs<D> * b + h];
+    for (int d = 0; d < (D == 3 ? 3 : 2); ++d) sn[(size_t)(D == 3 ? 3 : 2) * k + d] = A.c.n[3 * (size_t)k + d];
+    if (D == 3) {
+      for (int d = 0; d < 3; ++d) ss[3 * (size_t)k + d] = A.c.s[3 * (size_t)k + d];
+      sl[2 * (size_t)k] = A.c.la[k];
+      sl[2 * (size_t)k + 1] = A.c.lb[k];
+    }
+    sw[4 * (size_t)k] = A.c.wn[k];
+    sw[4 * (size_t)k + 1] = A.c.wt[k];
+    sw[4 * (size_t)k + 2] = A.c.mu[k];
+    sw[4 * (size_t)k + 3] = A.c.forcing[k];
+    for (int d = 0; d < 3; ++d) sr[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
+  }
+  __syncthreads();
+
+  auto rec = [&](uint32_t k, Rec<D, Real>& R) {
+    R.a = ea[k];
+    R.b = eb[k];
+    if (D == 3) {
+      for (int d = 0; d < 3; ++d) {
+        R.n[d] = sn[3 * (size_t)k + d];
+        R.s[d] = ss[3 * (size_t)k + d];
+      }
+      R.t[0] = R.n[1] * R.s[2] - R.n[2] * R.s[1];
+      R.t[1] = R.n[2] * R.s[0] - R.n[0] * R.s[2];
+      R.t[2] = R.n[0] * R.s[1] - R.n[1] * R.s[0];
+      R.la = sl[2 * (size_t)k];
+      R.lb = sl[2 * (size_t)k + 1];
+    } else {
+      R.n[0] = sn[2 * (size_t)k];
+      R.n[1] = sn[2 * (size_t)k + 1];
+      R.n[2] = 0;
+      R.t[0] = -R.n[1];
+      R.t[1] = R.n[0];
+      R.t[2] = 0;
+      R.s[0] = R.s[1] = R.s[2] = 0;
+      R.la = R.lb = 0;
+    }
+    R.wn = sw[4 * (size_t)k];
+    R.wt = sw[4 * (size_t)k + 1];
+    R.mu = sw[4 * (size_t)k + 2];
+    R.f = sw[4 * (size_t)k + 3];
+  };
+  // one contact: the k_gs update on LDS data (init: apply the stored impulse)
+  auto update = [&](uint32_t k, bool init) {
+    Rec<D, Real> R;
+    rec(k, R);
+    const bool hasb = R.b != kNoBody;
+    V* bv = sb;
+    const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
+    V va = bv[ia], wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
+    if (D == 3) wa = bv[ia + 1];
+    if (hasb) {
+      vb = bv[ib];
+      if (D == 3) wb = bv[ib + 1];
+    }
+    Real* rp = sr + 3 * (size_t)k;
+    const Real r[3] = {rp[0], rp[1], rp[2]};
+    Real dr[3];
+    if (init) {
+      dr[0] = r[0];
+      dr[1] = r[1];
+      dr[2] = r[2];
+    } else {
+      Real u[3], rn[3];
+      rel_vel<D, Real, Real>(R, va, wa, vb, wb, u);
+      local_solve<D, Real>(R, u, r, rn);
+      dr[0] = rn[0] - r[0];
+      dr[1] = rn[1] - r[1];
+      dr[2] = rn[2] - r[2];
+      rp[0] = rn[0];
+      rp[1] = rn[1];
+      if (D == 3) rp[2] = rn[2];
+    }
+    Real P[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) P[d] = R.n[d] * dr[0] + R.t[d] * dr[D - 1] + ((D == 3) ? R.s[d] * dr[1] : (Real)0);
+    va.x += va.w * P[0];
+    va.y += va.w * P[1];
+    va.z += va.w * P[2];
+    bv[ia] = va;
+    Real tau[3];
+    if (D == 3) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) tau[d] = dr[1] * R.t[d] - dr[2] * R.s[d];
+      const Real ka = -R.la * wa.w;
+      wa.x += ka * tau[0];
+      wa.y += ka * tau[1];
+      wa.z += ka * tau[2];
+      bv[ia + 1] = wa;
+    }
+    if (hasb) {
+      vb.x -= vb.w * P[0];
+      vb.y -= vb.w * P[1];
+      vb.z -= vb.w * P[2];
+      bv[ib] = vb;
+      if (D == 3) {
+        const Real kb = R.lb * wb.w;
+        wb.x -= kb * tau[0];
+        wb.y -= kb * tau[1];
+        wb.z -= kb * tau[2];
+        bv[ib + 1] = wb;
+      }
+    }
+  };
+  auto residual = [&]() -> double {
+    double m = 0.0;
+    for (uint32_t k = tid; k < C; k += kGSBlock) {
+      Rec<D, Real> R;
+      rec(k, R);
+      const V z = {0, 0, 0, 0};
+      const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
+      const V va = sb[ia];
+      const V wa = (D == 3) ? sb[ia + 1] : z;
+      const V vb = (R.b != kNoBody) ? sb[ib] : z;
+      const V wb = (D == 3 && R.b != kNoBody) ? sb[ib + 1] : z;
+      double u[3];
+      rel_vel<D, double, Real>(R, va, wa, vb, wb, u);
+      const double r[3] = {(double)sr[3 * (size_t)k], (double)sr[3 * (size_t)k + 1], (double)sr[3 * (size_t)k + 2]};
+      m = fmax(m, fb_err<D>((double)R.mu, r, u));
+    }
+    m = wave_reduce_max_f64(m);
+    if ((tid & 63) == 0) s_red[tid >> 6] = m;
+    __syncthreads();
+    double out = s_red[0];
+    for (int w = 1; w < kGSBlock / 64; ++w) out = fmax(out, s_red[w]);
+    __syncthreads();
+    return out;
+  };
+  // v = v0 + M^-1 H^T r: reset the owned bodies to v0, then apply every impulse colour by colour
+  auto apply_all = [&](bool reset) {
+    if (reset) {
+      for (uint32_t e = tid; e < 2 * C; e += kGSBlock) {
+        const uint32_t k = e >> 1;
+        const uint32_t owner = (e & 1) ? eb[k] : ea[k];
+        if (owner != e) continue;
+        const uint32_t slot = (e & 1) ? A.c.b[k] : A.c.a[k];
+        const uint32_t i = A.sid ? A.sid[slot] : slot;
+        V* rb = sb + (size_t)kBodyVecs<D> * e;
+        rb[0].x = (Real)A.v0[(size_t)D * i];
+        rb[0].y = (Real)A.v0[(size_t)D * i + 1];
+        rb[0].z = (D == 3) ? (Real)A.v0[(size_t)D * i + D - 1] : (Real)0;
+        if (D == 3) {
+          rb[1].x = (Real)A.w0[3 * (size_t)i];
+          rb[1].y = (Real)A.w0[3 * (size_t)i + 1];
+          rb[1].z = (Real)A.w0[3 * (size_t)i + 2];
+        }
+      }
+      __syncthreads();
+    }
+    for (uint32_t col = 0; col < ncol; ++col) {
+      for (uint32_t k = off[col] + tid; k < off[col + 1]; k += kGSBlock) update(k, true);
+      __syncthreads();
+    }
+  };
+
+  uint32_t max_iters = A.max_iters;
+  if (A.phase == 1) max_iters = max(1u, max_iters - max_iters / 4);
+  if (A.phase == 2) max_iters = (A.ctr->sweeps_fp32 < max_iters) ? max_iters - A.ctr->sweeps_fp32 : 0u;
+  const bool track_best = A.phase != 1;
+  if (A.apply_init) apply_all(false);
+  double err = residual();
+  double best = err;
+  if (track_best)
+    for (uint32_t k = tid; k < C; k += kGSBlock)
+      for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = sr[3 * (size_t)k + d];
+  double ref = best;
+  uint32_t ref_it = 0, it = 0;
+  if (!(err < A.tol)) {
+    for (it = 1; it <= max_iters; ++it) {
+      for (uint32_t col = 0; col < ncol; ++col) {
+        for (uint32_t k = off[col] + tid; k < off[col + 1]; k += kGSBlock) update(k, false);
+        __syncthreads();
+      }
+      if (A.eval_every > 0 && it % A.eval_every == 0) {
+        err = residual();
+        if (err < best) {
+          best = err;
+          if (track_best)
+            for (uint32_t k = tid; k < C; k += kGSBlock)
+              for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = sr[3 * (size_t)k + d];
+        }
+        if (err < A.tol) break;
+        if (A.phase == 1 && it - ref_it >= 256) {
+          if (best > 0.99 * ref) break;
+          ref = best;
+          ref_it = it;
+        }
+      }
+    }
+    if (it > max_iters && track_best) {
+      // not converged: the best iterate and its velocities
+      __syncthreads();
+      for (uint32_t k = tid; k < C; k += kGSBlock)
+        for (int d = 0; d < 3; ++d) sr[3 * (size_t)k + d] = A.rbest[3 * (size_t)k + d];
+      __syncthreads();
+      apply_all(true);
+    }
+    if (it > max_iters) it = max_iters;
+  }
+  __syncthreads();
+  // results back: impulses, owned bodies; then the owner map back to all ones
+  V* gbw = reinterpret_cast<V*>(A.body);
+  for (uint32_t k = tid; k < C; k += kGSBlock) {
+    for (int d = 0; d < 3; ++d) A.c.r[3 * (size_t)k + d] = sr[3 * (size_t)k + d];
+    if (ea[k] == 2 * k)
+      for (int h = 0; h < kBodyVecs<D>; ++h) gbw[(size_t)kBodyVecs<D> * A.c.a[k] + h] = sb[(size_t)kBodyVecs<D> * ea[k] + h];
+    if (eb[k] == 2 * k + 1)
+      for (int h = 0; h < kBodyVecs<D>; ++h) gbw[(size_t)kBodyVecs<D> * A.c.b[k] + h] = sb[(size_t)kBodyVecs<D> * eb[k] + h];
+  }
+  __syncthreads();
+  for (uint32_t k = tid; k < C; k += kGSBlock) {
+    if (ea[k] == 2 * k) A.lmap[A.c.a[k]] = kNoBody;
+    if (eb[k] == 2 * k + 1) A.lmap[A.c.b[k]] = kNoBody;
+  }
+  if (tid == 0) {
+    if (A.phase == 1) {
+      A.ctr->sweeps_fp32 = it;
+      A.ctr->error_fp32 = best;
+    }
+    A.ctr->sweeps += it;
+    A.ctr->error = best;
+  }
+}
+
 // Body records [v | minv] ([omega | Iinv]) by slot, in both precisions.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_body_init(uint32_t n, const uint32_t* __restrict__ sid,
@@ -482,10 +761,22 @@ __global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, co
 // Persistent launch sized by the widest colour and the barrier/pass trade-off of
 // persistent_blocks (the barrier's cost grows with the workgroups): one workgroup (a
 // workgroup barrier, no device fence) for narrow colours, more as colours widen.
+// Largest dynamic LDS of the single-workgroup solve (gfx950: 160 KB per workgroup).
+constexpr size_t kGSLdsMax = 150 * 1024;
+
 template <int D, typename Real>
 static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, uint32_t max_colour) {
   const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(max_colour, kGSBlock, max_blocks);
   const int blocks = std::max(1, std::min(max_blocks, want));
+  const size_t lds = LdsLayout<D, Real>::bytes(A.C);
+  if (blocks == 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap) {
+    // the whole solve in one workgroup's LDS
+    hipError_t e = hipFuncSetAttribute((const void*)k_gs_lds<D, Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    k_gs_lds<D, Real><<<1, kGSBlock, lds, c->stream>>>(A);
+    return hipGetLastError();
+  }
   void* args[] = {&A};
   return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
 }
@@ -515,6 +806,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
     GSArgs<float> A{c->d.c32, c->d.bv32, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
                     reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
+    A.lmap = c->d.lmap;
     e = coop_gs<D, float>(c, A, c->gs_blocks, max_colour);
     if (e != hipSuccess) return e;
     timing_mark(c, "gs_fp32");
@@ -527,6 +819,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
   GSArgs<double> B{c->d.c64, c->d.bv64, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
+  B.lmap = c->d.lmap;
   return coop_gs<D, double>(c, B, c->gs_blocks, max_colour);
 }
 

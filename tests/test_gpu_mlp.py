@@ -65,3 +65,44 @@ def test_device_training_fits_a_smooth_map(ctx):
     q = ctx.warm_model()
     l_ref, _ = mlp.loss_and_grad(q, X[:512], T[:512], emulate_bf16=True)
     assert abs(l_ref - loss) <= 5e-3 * max(l_ref, 1e-12) + 1e-6
+
+
+def test_learned_warm_start_cuts_sweeps_on_a_settled_pile():
+    """On a settled pile, the perceptron trained on the device on the pile's own
+    (feature, converged impulse) samples warm-starts the solve: every step still reaches
+    the tolerance, with far fewer sweeps than cold starts from the same states."""
+    from scisim_amd import scenes, warmstart
+    s = scenes.spheres_pile(n=3000, seed=4)
+    ctx = scx.Context(3, 3000, max_planes=8)
+    ctx.set_planes(s["px"], s["pn"])
+    ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s["inertia"])
+    base = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, tol=1e-6, max_iters=5000, integrator=1)
+
+    def params(mode):
+        return scx.step_params(precision=scx.FP32_POLISH, warm_start=mode, **base)
+
+    for _ in range(300):  # settle under the transported cache
+        ctx.step(params(scx.WARM_TRANSPORT))
+    Xs, Ts = [], []
+    for _ in range(30):
+        ctx.step(params(scx.WARM_TRANSPORT))
+        X, T = ctx.samples()
+        Xs.append(X)
+        Ts.append(T)
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    ctx.set_warm_model(warmstart.init_params(seed=2))
+    loss = ctx.train_warm_model(X, T, epochs=20, batch=1024, lr=1e-3, seed=3)
+    q, v, w = ctx.get_state()
+    sweeps = {}
+    for name, mode in (("cold", scx.WARM_NONE), ("learned", scx.WARM_LEARNED)):
+        ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s["inertia"])
+        ctx.step(params(scx.WARM_TRANSPORT))
+        sw = []
+        for _ in range(8):
+            st = ctx.step(params(mode))
+            assert st["succeeded"], (name, st)
+            sw.append(st["sweeps"])
+        sweeps[name] = sw
+    ctx.close()
+    _say("samples", X.shape[0], "loss", loss, "sweeps", sweeps)
+    assert sum(sweeps["learned"]) * 2 <= sum(sweeps["cold"])
