@@ -12,6 +12,11 @@ Workloads (synthetic, seeded):
   cfg3: cfg2 with the learned warm start: samples of `--collect` steps, the perceptron
         trained on them on the device (untimed), then the timed steps with it; the cold
         and transported-cache warm starts are measured from the same state for comparison.
+  cfg4: an ensemble of 512 independent poured piles stepped together (scx_set_scenes):
+        body counts log-uniform in [500, 20000], radii U[0.03, 0.07], mu U[0.1, 0.8] per
+        scene, Verlet, dt = 0.005; settled under the transported cache, then the learned
+        warm start trained as in cfg3. Every scene completes each step together, so the
+        per-scene step latency is the step's wall time (p50/p99 over the timed steps).
 
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
 narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final
@@ -42,6 +47,16 @@ def make_scene(cfg, seed):
         desc = {"workload": "ball2d_rsa_1M", "bodies": 1048576, "dim": 2, "radii": "U[0.02,0.06]",
                 "packing": "RSA phi=0.55", "planes": 4, "mu": 0.3, "CoR": 0.0, "dt": 0.005,
                 "integrator": "symplectic_euler", "tol": 1e-6}
+    elif cfg == "cfg4":
+        s, body_off, plane_off, mus, counts = scenes.ensemble_piles(nscenes=512, seed=seed)
+        s["ensemble"] = (body_off, plane_off, mus)
+        prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.5, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1,
+                   integrator=1)
+        desc = {"workload": "spheres3d_ensemble_512", "scenes": 512, "bodies": int(body_off[-1]),
+                "bodies_per_scene": "loguniform[500,20000] (min %d, max %d)" % (counts.min(), counts.max()),
+                "dim": 3, "radii": "U[0.03,0.07]", "mu": "U[0.1,0.8] per scene",
+                "packing": "poured piles (sequential deposition), settled", "planes_per_scene": 5, "CoR": 0.0,
+                "dt": 0.005, "integrator": "verlet", "tol": 1e-6}
     else:
         s = scenes.spheres_pile(n=262144, rmed=0.05, sigma=0.25, seed=seed)
         prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1,
@@ -53,7 +68,7 @@ def make_scene(cfg, seed):
     return s, prm, desc
 
 
-def train_learned_warm_start(ctx, s, prm, precision, args, log):
+def train_learned_warm_start(ctx, s, prm, precision, args, log, compare=True):
     """cfg3: samples of `collect` steps under the transported cache (a quarter of the
     contacts of each step, seeded), the perceptron trained on them on the device, then the
     sweeps and step times of the cold, transported and learned warm starts over the same
@@ -63,10 +78,11 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log):
     rng = np.random.default_rng(args.seed)
     tp = scx.step_params(precision=precision, **dict(prm, warm_start=scx.WARM_TRANSPORT))
     Xs, Ts = [], []
+    frac = 0.25 if s["r"].size <= 300000 else 0.05  # samples kept per step (host memory)
     for k in range(args.collect):
         ctx.step(tp)
         X, T = ctx.samples()
-        keep = rng.random(X.shape[0]) < 0.25
+        keep = rng.random(X.shape[0]) < frac
         Xs.append(X[keep])
         Ts.append(T[keep])
     X, T = np.concatenate(Xs), np.concatenate(Ts)
@@ -78,16 +94,23 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log):
     q, v, w = ctx.get_state()
     out = {"training_samples": int(X.shape[0]), "collect_steps": args.collect, "epochs": 12, "train_s": train_s,
            "final_mse": loss}
-    for name, mode in (("none", scx.WARM_NONE), ("transport", scx.WARM_TRANSPORT), ("learned", scx.WARM_LEARNED)):
+    if not compare:
+        return out
+    # every mode starts from the same state, after one transported-cache step that primes
+    # the cache (a cold step leaves the cache empty, so the cold start runs last)
+    for name, mode in (("transport", scx.WARM_TRANSPORT), ("learned", scx.WARM_LEARNED), ("none", scx.WARM_NONE)):
         ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
-        ctx.step(tp)  # the same cache for every mode
-        sw = []
+        ctx.step(tp)
+        sw, ok = [], 0
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            sw.append(ctx.step(scx.step_params(precision=precision, **dict(prm, warm_start=mode)))["sweeps"])
+            st = ctx.step(scx.step_params(precision=precision, **dict(prm, warm_start=mode)))
+            sw.append(st["sweeps"])
+            ok += int(st["succeeded"])
         dt = time.perf_counter() - t0
-        out[name] = {"mean_sweeps": float(np.mean(sw)), "body_steps_per_s": s["r"].size * args.steps / dt}
-        log(name, out[name])
+        out[name] = {"mean_sweeps": float(np.mean(sw)), "converged_steps": ok,
+                     "body_steps_per_s": s["r"].size * args.steps / dt}
+        log(name, out[name], sw)
     ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
     ctx.step(tp)
     return out
@@ -95,6 +118,22 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log):
 
 def cpu_baseline(s, prm, budget_s, max_steps):
     import oracle
+    if "ensemble" in s:
+        # one step of each scene alone, scene after scene, within the budget
+        body_off, plane_off, mus = s["ensemble"]
+        t0 = time.perf_counter()
+        done = k = 0
+        while k < len(mus) and (time.perf_counter() - t0) < budget_s:
+            b0, b1, p0, p1 = int(body_off[k]), int(body_off[k + 1]), int(plane_off[k]), int(plane_off[k + 1])
+            sim = oracle.Sim(3, s["q"][3 * b0:3 * b1], s["v"][3 * b0:3 * b1], s["r"][b0:b1], s["m"][b0:b1],
+                             s["px"][p0:p1], s["pn"][p0:p1], inertia=s["inertia"][b0:b1])
+            sim.step(order=0, **dict(prm, mu=float(mus[k]), warm_start=0))
+            done += b1 - b0
+            k += 1
+        dt = time.perf_counter() - t0
+        return {"value": done / dt, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                "sample": "FP64 oracle, natural-order GS, one step of each of the first %d scenes alone (%d bodies, "
+                          "unsettled start state), %.1f s" % (k, done, dt)}
     sim = oracle.Sim(s["dim"], s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"], inertia=s.get("inertia"))
     t0 = time.perf_counter()
     k = 0
@@ -112,10 +151,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2", "cfg3"])
+    ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2", "cfg3", "cfg4"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--settle", type=int, default=300, help="untimed settling steps of the pile (cfg2, cfg3)")
-    ap.add_argument("--collect", type=int, default=40, help="steps whose samples train the warm start (cfg3)")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="untimed settling steps of the piles (default: cfg2, cfg3 300; cfg4 60)")
+    ap.add_argument("--collect", type=int, default=None,
+                    help="steps whose samples train the warm start (default: cfg3 40, cfg4 10)")
     ap.add_argument("--warm", default="none", choices=["none", "cache", "normal", "transport"],
                     help="impulse cache warm start (ImpulsesToCache)")
     ap.add_argument("--seed", type=int, default=1)
@@ -153,12 +194,18 @@ def main():
     dim = s["dim"]
     n = s["r"].size
     log("scene ready", desc["workload"], n)
-    ctx = scx.Context(dim, n, max_planes=8, device=0)
+    ctx = scx.Context(dim, n, max_planes=max(8, s["px"].size // dim), device=0)
     ctx.set_planes(s["px"], s["pn"])
     ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s.get("inertia"))
+    if "ensemble" in s:
+        ctx.set_scenes(*s["ensemble"])
     precision = scx.FP32_POLISH if args.precision == "fp32" else scx.FP64
     it = 1
-    pile = args.config in ("cfg2", "cfg3")
+    pile = args.config in ("cfg2", "cfg3", "cfg4")
+    if args.settle is None:
+        args.settle = 60 if args.config == "cfg4" else 300
+    if args.collect is None:
+        args.collect = 10 if args.config == "cfg4" else 40
     if pile and args.settle > 0:
         # the deposited pile settles under the transported impulse cache (untimed)
         sp = scx.step_params(precision=precision, **dict(prm, max_iters=1000, warm_start=scx.WARM_TRANSPORT))
@@ -174,8 +221,8 @@ def main():
     if args.eval_every is not None:
         prm["eval_every"] = args.eval_every
     comparison = None
-    if args.config == "cfg3":
-        comparison = train_learned_warm_start(ctx, s, prm, precision, args, log)
+    if args.config in ("cfg3", "cfg4"):
+        comparison = train_learned_warm_start(ctx, s, prm, precision, args, log, compare=args.config == "cfg3")
         args.warm = "learned"
     prm["warm_start"] = {"none": scx.WARM_NONE, "cache": scx.WARM_CACHE, "normal": scx.WARM_NORMAL,
                          "transport": scx.WARM_TRANSPORT, "learned": scx.WARM_LEARNED}[args.warm]
@@ -189,10 +236,14 @@ def main():
     pairs_bytes = gs_bytes = 0.0
     contacts, sweeps = [], []
     fp32 = precision == scx.FP32_POLISH
+    step_ms = []
     t0 = time.perf_counter()
     for k in range(args.steps):
+        ts = time.perf_counter()
         st = ctx.step(params)
-        for name, ms in ctx.timing():
+        step_ms.append(1000.0 * (time.perf_counter() - ts))
+        tm = ctx.timing()
+        for name, ms in tm:
             phase_ms[name] = phase_ms.get(name, 0.0) + ms
         C = st["ncontacts"]
         for write_pass in (False, True):
@@ -204,7 +255,7 @@ def main():
         gs_bytes += rl.gs_launch_bytes(dim, C, s64, s64 + 1, False)
         contacts.append(C)
         sweeps.append(st["sweeps"])
-        log("step", k, st)
+        log("step", k, st, " ".join("%s %.3f" % (name, ms) for name, ms in tm if ms > 0.02))
     wall = time.perf_counter() - t0
     ctx.set_timing(False)
     ms_per_step = 1000.0 * wall / args.steps
@@ -218,6 +269,12 @@ def main():
     else:
         kernel, launches, kbytes, kms = "k_gs (coloured Gauss-Seidel, persistent)", 1 + int(fp32), gs_bytes, gs_ms
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    if "ensemble" in s:
+        cnt, err = ctx.scene_stats()
+        desc["step_latency_ms_p50"] = float(np.percentile(step_ms, 50))
+        desc["step_latency_ms_p99"] = float(np.percentile(step_ms, 99))
+        desc["scene_contacts_p50_p99"] = [float(np.percentile(cnt, 50)), float(np.percentile(cnt, 99))]
+        desc["scene_residual_max"] = float(err.max())
     out = {
         "metric": "body-steps/sec at FB residual<=1e-6",
         "value": value,

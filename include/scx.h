@@ -71,7 +71,7 @@ typedef struct scx_config {
   int32_t dim;          /* 2 (ball2d) or 3 (rigidbody3d spheres) */
   int32_t device;       /* HIP device ordinal */
   uint32_t max_bodies;  /* body capacity */
-  uint32_t max_planes;  /* static plane capacity (<= 64) */
+  uint32_t max_planes;  /* static plane capacity (grows on demand; at most 64 per scene) */
   uint32_t max_contacts; /* contact capacity; 0 = 8 * max_bodies */
   uint32_t max_pairs;    /* broad-phase pair capacity for scx_potential_overlaps; 0 = 16 * max_bodies */
 } scx_config;
@@ -123,8 +123,25 @@ int scx_potential_overlaps(scx_ctx* ctx, uint32_t n, const double* aabb_min, con
  * r radii; m masses; inertia = isotropic sphere inertia (3D; NULL = 2/5 m r^2). */
 int scx_set_bodies(scx_ctx* ctx, uint32_t n, const double* q, const double* v, const double* omega,
                    const double* r, const double* m, const double* inertia);
-/* Static planes: point x and normal n (normalised here as StaticPlane does), dim each. */
+/* Static planes: point x and normal n (normalised here as StaticPlane does), dim each.
+ * At most 64 for one scene; an ensemble (scx_set_scenes) has at most 64 per scene. */
 int scx_set_planes(scx_ctx* ctx, uint32_t nplanes, const double* x, const double* n);
+
+/* Ensemble of independent scenes stepped together (no counterpart in the reference,
+ * which runs one simulation per process; this is for callers that batch many small
+ * scenes onto one GPU). Scene k owns bodies [body_off[k], body_off[k+1]) of the arrays
+ * given to scx_set_bodies and static planes [plane_off[k], plane_off[k+1]) of those given
+ * to scx_set_planes (at most 64), and has friction coefficient mu[k] (scx_step_params.mu
+ * is then unused). Bodies of different scenes never interact; within a scene the active
+ * set, its order and the contact data are those of the scene alone. The Gauss-Seidel
+ * solve runs until every scene meets the tolerance (one residual, the max over all
+ * contacts). Call after scx_set_bodies and scx_set_planes; a new body or plane count
+ * clears the ensemble; nscenes = 0 returns to one scene. */
+int scx_set_scenes(scx_ctx* ctx, uint32_t nscenes, const uint32_t* body_off, const uint32_t* plane_off,
+                   const double* mu);
+/* Per-scene results of the last step of an ensemble: contacts and residual (max over the
+ * scene's contacts of ||FB||^2, in FP64). Either array may be NULL. */
+int scx_get_scene_stats(scx_ctx* ctx, uint32_t nscenes, uint32_t* ncontacts, double* error);
 int scx_get_state(scx_ctx* ctx, double* q, double* v, double* omega);
 
 /* One GeometricImpactFrictionMap step on the resident state. stats may be NULL. */

@@ -35,7 +35,7 @@ EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_po
            "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
            "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
            "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
-           "scx_train_warm_model"]
+           "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats"]
 
 MLP_PARAMS = 19075  # SCX_MLP_PARAMS
 WARM_LEARNED = 4
@@ -97,6 +97,8 @@ def lib():
                                              ctypes.POINTER(ctypes.c_uint64)]
         L.scx_set_bodies.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, _pd, _pd, _pd, _pd]
         L.scx_set_planes.argtypes = [_vp, ctypes.c_uint32, _pd, _pd]
+        L.scx_set_scenes.argtypes = [_vp, ctypes.c_uint32, _pu32, _pu32, _pd]
+        L.scx_get_scene_stats.argtypes = [_vp, ctypes.c_uint32, _pu32, _pd]
         L.scx_get_state.argtypes = [_vp, _pd, _pd, _pd]
         L.scx_step.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.POINTER(ScxStepStats)]
         L.scx_run.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.c_uint32, ctypes.POINTER(ScxStepStats)]
@@ -150,6 +152,7 @@ class Context:
                 self._h = _vp()
             raise ScxError("scx_create failed (%d): %s" % (rc, msg.decode()))
         self.n = 0
+        self.nscenes = 0
 
     def close(self):
         if self._h:
@@ -186,13 +189,37 @@ class Context:
 
     def set_bodies(self, q, v, r, m, omega=None, inertia=None):
         q, v, r, m = _f64(q), _f64(v), _f64(r), _f64(m)
+        if r.size != self.n:
+            self.nscenes = 0
         self.n = r.size
         self._check(lib().scx_set_bodies(self._h, self.n, _p(q), _p(v), _p(_f64(omega)), _p(r), _p(m),
                                          _p(_f64(inertia))), "scx_set_bodies")
 
     def set_planes(self, x, n):
         x, n = _f64(x), _f64(n)
-        self._check(lib().scx_set_planes(self._h, x.size // self.dim, _p(x), _p(n)), "scx_set_planes")
+        if x.size // self.dim != getattr(self, "nplanes", 0):
+            self.nscenes = 0
+        self.nplanes = x.size // self.dim
+        self._check(lib().scx_set_planes(self._h, self.nplanes, _p(x), _p(n)), "scx_set_planes")
+
+    def set_scenes(self, body_off, plane_off, mu):
+        """Ensemble of independent scenes: scene k owns bodies body_off[k]:body_off[k+1] and
+        planes plane_off[k]:plane_off[k+1], with friction coefficient mu[k]; an empty
+        body_off returns to one scene."""
+        bo = np.ascontiguousarray(body_off, dtype=np.uint32)
+        po = np.ascontiguousarray(plane_off, dtype=np.uint32)
+        mu = _f64(mu)
+        ns = max(0, bo.size - 1)
+        self.nscenes = ns
+        self._check(lib().scx_set_scenes(self._h, ns, _p(bo, _pu32), _p(po, _pu32), _p(mu)), "scx_set_scenes")
+
+    def scene_stats(self):
+        """(contacts, residual) per scene of the last step of an ensemble."""
+        ns = self.nscenes
+        cnt = np.zeros(ns, dtype=np.uint32)
+        err = np.zeros(ns)
+        self._check(lib().scx_get_scene_stats(self._h, ns, _p(cnt, _pu32), _p(err)), "scx_get_scene_stats")
+        return cnt, err
 
     def get_state(self):
         q = np.empty(self.dim * self.n)

@@ -192,14 +192,16 @@ int scn_rsa3d(uint32_t n, double rmed, double sigma, double rclamp_lo, double rc
 // below 1e-7 r. A sphere so comes to rest in a pocket of (typically) three supports,
 // which gives a pile with about six contacts per sphere. Each sphere is finally lowered
 // by `sink` so that it overlaps its supports slightly and its contacts are active at the
-// first step. Writes q (3n), r (n) and the pile height.
-int scn_deposit3d(uint32_t n, double rmed, double sigma, double rclamp_lo, double rclamp_hi, double base,
-                  uint64_t seed, double sink, double* q, double* r, double* height_out) {
+// first step. Radii: uniform_radii = 0, rmed exp(sigma Z) clamped to [rclamp_lo,
+// rclamp_hi]; otherwise uniform in [rclamp_lo, rclamp_hi]. Writes q (3n), r (n) and the
+// pile height.
+int scn_deposit3d(uint32_t n, double rmed, double sigma, double rclamp_lo, double rclamp_hi, int uniform_radii,
+                  double base, uint64_t seed, double sink, double* q, double* r, double* height_out) {
   Rng rng(seed);
   std::vector<double> radii(n);
   double vol = 0.0, rmax = 0.0, dsum = 0.0;
   for (uint32_t i = 0; i < n; ++i) {
-    double x = rmed * std::exp(sigma * rng.normal());
+    double x = uniform_radii ? rclamp_lo + (rclamp_hi - rclamp_lo) * rng.uniform() : rmed * std::exp(sigma * rng.normal());
     x = std::min(rclamp_hi, std::max(rclamp_lo, x));
     radii[i] = x;
     rmax = std::max(rmax, x);

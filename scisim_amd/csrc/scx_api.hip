@@ -165,6 +165,7 @@ int alloc_contacts(scx_ctx* c, uint32_t C) {
   HIPC(dalloc(&d.cwn, C));
   HIPC(dalloc(&d.cwt, C));
   HIPC(dalloc(&d.cforcing, C));
+  HIPC(dalloc(&d.cmu, C));
   HIPC(dalloc(&d.calpha, C));
   HIPC(dalloc(&d.cbeta, (size_t)(D - 1) * C));
   HIPC(dalloc(&d.crinit, (size_t)D * C));
@@ -198,12 +199,16 @@ void free_all(scx_ctx* c) {
   DeviceBuffers& d = c->d;
   double* dbl[] = {d.q, d.v, d.w, d.r, d.m, d.inertia, d.q0, d.v0, d.w0, d.q1p, d.v2, d.w2, d.bv64,
                    d.px, d.pn, d.aabb_min, d.aabb_max, d.red, d.basis, d.cla, d.clb, d.cwn, d.cwt, d.cforcing,
-                   d.calpha, d.cbeta, d.crinit, d.rbest, d.partial, d.prev_r};
+                   d.calpha, d.cbeta, d.crinit, d.rbest, d.partial, d.prev_r, d.scene_mu, d.scene_shift,
+                   d.scene_err, d.cmu};
   for (double* p : dbl)
     if (p) hipFree(p);
   if (d.bv32) hipFree(d.bv32);
   dfree(&d.slot_of);
   dfree(&d.lmap);
+  dfree(&d.bscene);
+  dfree(&d.scene_poff);
+  dfree(&d.scene_cnt);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -293,20 +298,23 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   std::swap(d.order, d.order_next);
   HIPC(launch_slots(c, n));
   // plane contacts, plane-major (Ball2DSim.cpp:747-764 / RigidBody3DSim.cpp:1414-1502)
+  // (ensembles: buckets are scene-local plane indices, so every scene's plane contacts
+  // keep their own plane-major order)
   uint32_t npl = 0;
-  if (c->nplanes > 0) {
+  const uint32_t nbuckets = c->nscenes ? c->scene_planes : c->nplanes;
+  if (nbuckets > 0) {
     HIPC(launch_plane_masks(c, n));
-    HIPC(launch_multibucket(c, d.pmask, n, c->nplanes, d.perm, c->cap_perm, d.coloff));
+    HIPC(launch_multibucket(c, d.pmask, n, nbuckets, d.perm, c->cap_perm, d.coloff));
   }
   timing_mark(c, "planes");
   // sync point: sizes of the active set
   HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
   HIPC(hipMemcpyAsync(d.host_u32, d.pcount + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  if (c->nplanes > 0)
-    HIPC(hipMemcpyAsync(d.host_u32 + 1, d.coloff + c->nplanes, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (nbuckets > 0)
+    HIPC(hipMemcpyAsync(d.host_u32 + 1, d.coloff + nbuckets, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
   const uint32_t nbb = d.host_u32[0];
-  if (c->nplanes > 0) npl = d.host_u32[1];
+  if (nbuckets > 0) npl = d.host_u32[1];
   const uint64_t nentries = d.ctr_host->nentries;
   if ((d.ctr_host->overflow & 1u) || nentries > c->cap_entries) {
     std::swap(d.q, d.q0);
@@ -327,9 +335,9 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     return SCX_E_CAPACITY;
   }
   const uint32_t C = (uint32_t)C64;
-  uint32_t max_colour = 0;
   if (C > 0) {
     HIPC(launch_contact_setup(c, nbb, npl, p->CoR, p->mu));
+    HIPC(launch_contact_mu(c, C));
     // warm start from the impulse cache (GeometricImpactFrictionMap.cpp:306); the learned
     // warm start reads the transported cache as an input and predicts the impulses
     if (p->warm_start != SCX_WARM_NONE) {
@@ -355,13 +363,12 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     // second sync point: colour sizes size the persistent sweep launch
     HIPC(hipMemcpyAsync(d.host_u32 + 2, d.coloff, sizeof(uint32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
-    for (int b = 0; b < kMaxColours; ++b) max_colour = std::max(max_colour, d.host_u32[3 + b] - d.host_u32[2 + b]);
     timing_mark(c, "colour");
-    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, nullptr,
+    HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, c->nscenes ? d.cmu : nullptr,
                                p->warm_start != SCX_WARM_NONE ? d.crinit : nullptr));
     timing_mark(c, "gather");
   }
-  HIPC(launch_solve(c, C, max_colour, p));
+  HIPC(launch_solve(c, C, d.host_u32 + 2, p));
   timing_mark(c, "gs_fp64");
   HIPC(launch_finish(c, C, p));
   // cache this step's impulses (GeometricImpactFrictionMap.cpp:419); an empty active set
@@ -421,6 +428,9 @@ int check_params(scx_ctx* c, const scx_step_params* p) {
   // GeometricImpactFrictionMap.cpp:258-262: "Caching in 3 space not yet supported."
   if (p->warm_start == SCX_WARM_CACHE && c->dim == 3)
     return fail(c, SCX_E_ARG, "normal and friction impulse caching is 2D only (use SCX_WARM_NORMAL)");
+  if (c->nscenes == 0 && c->nplanes > kMaxScenePlanes)
+    return fail(c, SCX_E_ARG, "at most %u static planes per scene (use scx_set_scenes for ensembles)",
+                kMaxScenePlanes);
   return SCX_OK;
 }
 
@@ -435,7 +445,8 @@ const char* scx_last_error(const scx_ctx* ctx) { return ctx ? ctx->err.c_str() :
 int scx_create(const scx_config* cfg, scx_ctx** out) {
   if (!out) return SCX_E_ARG;
   *out = nullptr;
-  if (!cfg || (cfg->dim != 2 && cfg->dim != 3) || cfg->max_bodies == 0 || cfg->max_planes > 64) return SCX_E_ARG;
+  if (!cfg || (cfg->dim != 2 && cfg->dim != 3) || cfg->max_bodies == 0 || cfg->max_planes > kMaxPlanesTotal)
+    return SCX_E_ARG;
   scx_ctx* c = new scx_ctx;
   c->dim = cfg->dim;
   c->device = cfg->device;
@@ -557,7 +568,10 @@ int scx_set_bodies(scx_ctx* c, uint32_t n, const double* q, const double* v, con
   // The impulse cache belongs to the scene, as the reference's constraint cache belongs to
   // the simulation: new positions and velocities of the same bodies (a caller feeding
   // each step's q0, v0) keep it, a different body count empties it.
-  if (n != c->n) d.prev_count = 0;
+  if (n != c->n) {
+    d.prev_count = 0;
+    c->nscenes = 0;  // an ensemble's layout refers to the old bodies
+  }
   c->n = n;
   c->has_bodies = true;
   c->order_valid = false;
@@ -567,7 +581,7 @@ int scx_set_bodies(scx_ctx* c, uint32_t n, const double* q, const double* v, con
 
 int scx_set_planes(scx_ctx* c, uint32_t nplanes, const double* x, const double* nrm) {
   if (!c) return SCX_E_ARG;
-  if (nplanes > 64) return fail(c, SCX_E_ARG, "at most 64 static planes");
+  if (nplanes > kMaxPlanesTotal) return fail(c, SCX_E_ARG, "at most %u static planes", kMaxPlanesTotal);
   if (nplanes > 0 && (!x || !nrm)) return fail(c, SCX_E_ARG, "null plane arrays");
   HIPC(hipSetDevice(c->device));
   const int D = c->dim;
@@ -590,7 +604,128 @@ int scx_set_planes(scx_ctx* c, uint32_t nplanes, const double* x, const double* 
     HIPC(hipMemcpyAsync(c->d.pn, nn.data(), sizeof(double) * D * nplanes, hipMemcpyHostToDevice, c->stream));
   }
   HIPC(hipStreamSynchronize(c->stream));
+  if (nplanes != c->nplanes) c->nscenes = 0;
   c->nplanes = nplanes;
+  return SCX_OK;
+}
+
+int scx_set_scenes(scx_ctx* c, uint32_t nscenes, const uint32_t* body_off, const uint32_t* plane_off,
+                   const double* mu) {
+  if (!c) return SCX_E_ARG;
+  if (nscenes == 0) {
+    c->nscenes = 0;
+    return SCX_OK;
+  }
+  if (!c->has_bodies) return fail(c, SCX_E_STATE, "scx_set_bodies must be called before scx_set_scenes");
+  if (!body_off || !plane_off || !mu) return fail(c, SCX_E_ARG, "scx_set_scenes: null array");
+  if (body_off[0] != 0 || body_off[nscenes] != c->n)
+    return fail(c, SCX_E_ARG, "scene body offsets must run from 0 to the body count %u", c->n);
+  if (plane_off[0] != 0 || plane_off[nscenes] != c->nplanes)
+    return fail(c, SCX_E_ARG, "scene plane offsets must run from 0 to the plane count %u", c->nplanes);
+  uint32_t most = 0;
+  for (uint32_t k = 0; k < nscenes; ++k) {
+    if (body_off[k + 1] < body_off[k] || plane_off[k + 1] < plane_off[k])
+      return fail(c, SCX_E_ARG, "scene %u: offsets must not decrease", k);
+    if (plane_off[k + 1] - plane_off[k] > kMaxScenePlanes)
+      return fail(c, SCX_E_ARG, "scene %u: at most %u static planes per scene", k, kMaxScenePlanes);
+    if (!(mu[k] >= 0.0)) return fail(c, SCX_E_ARG, "scene %u: mu must be non-negative", k);
+    most = std::max(most, plane_off[k + 1] - plane_off[k]);
+  }
+  HIPC(hipSetDevice(c->device));
+  DeviceBuffers& d = c->d;
+  hipStream_t s = c->stream;
+  const int D = c->dim;
+  const uint32_t n = c->n;
+  // binning shifts: each scene's region (bounds of its bodies' spheres, widened) placed
+  // on shelves along x, the shelves stacked along the last axis, with a gap between
+  // regions; the grid then keeps scenes apart and stays about as dense as the scenes
+  std::vector<double> q((size_t)D * n), r(n);
+  HIPC(hipMemcpyAsync(q.data(), d.q, sizeof(double) * D * n, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(r.data(), d.r, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  std::vector<double> lo((size_t)3 * nscenes, 0.0), hi((size_t)3 * nscenes, 0.0);
+  double rmax = 0.0;
+  for (uint32_t k = 0; k < nscenes; ++k) {
+    double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = body_off[k]; i < body_off[k + 1]; ++i) {
+      for (int a = 0; a < D; ++a) {
+        l[a] = std::min(l[a], q[(size_t)D * i + a] - r[i]);
+        h[a] = std::max(h[a], q[(size_t)D * i + a] + r[i]);
+      }
+      rmax = std::max(rmax, r[i]);
+    }
+    for (int a = 0; a < D; ++a) {
+      if (!(l[a] <= h[a])) l[a] = h[a] = 0.0;  // a scene without bodies
+      const double w = 0.1 * (h[a] - l[a]);
+      lo[3 * (size_t)k + a] = l[a] - w;
+      hi[3 * (size_t)k + a] = h[a] + w;
+    }
+  }
+  const double gap = 4.0 * rmax + 1e-3;
+  const int ax = 0, az = D - 1;  // shelf axis, stacking axis
+  std::vector<uint32_t> idx(nscenes);
+  double area = 0.0, widest = 0.0;
+  for (uint32_t k = 0; k < nscenes; ++k) {
+    idx[k] = k;
+    const double wx = hi[3 * (size_t)k + ax] - lo[3 * (size_t)k + ax] + gap;
+    const double wz = hi[3 * (size_t)k + az] - lo[3 * (size_t)k + az] + gap;
+    area += wx * wz;
+    widest = std::max(widest, wx);
+  }
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+    return hi[3 * (size_t)a + az] - lo[3 * (size_t)a + az] > hi[3 * (size_t)b + az] - lo[3 * (size_t)b + az];
+  });
+  const double row = std::max(widest, 1.2 * std::sqrt(area));
+  std::vector<double> shift((size_t)3 * nscenes, 0.0);
+  double x = 0.0, z = 0.0, depth = 0.0;
+  for (uint32_t k : idx) {
+    const double wx = hi[3 * (size_t)k + ax] - lo[3 * (size_t)k + ax] + gap;
+    const double wz = hi[3 * (size_t)k + az] - lo[3 * (size_t)k + az] + gap;
+    if (x > 0.0 && x + wx > row) {
+      z += depth;
+      x = 0.0;
+      depth = 0.0;
+    }
+    for (int a = 0; a < D; ++a) shift[3 * (size_t)k + a] = -lo[3 * (size_t)k + a];
+    shift[3 * (size_t)k + ax] += x;
+    if (az != ax) shift[3 * (size_t)k + az] += z;
+    x += wx;
+    depth = std::max(depth, wz);
+  }
+  std::vector<uint32_t> bs(n);
+  for (uint32_t k = 0; k < nscenes; ++k)
+    for (uint32_t i = body_off[k]; i < body_off[k + 1]; ++i) bs[i] = k;
+  if (nscenes > c->cap_scenes) {
+    HIPC(dalloc(&d.scene_poff, (size_t)nscenes + 1));
+    HIPC(dalloc(&d.scene_mu, nscenes));
+    HIPC(dalloc(&d.scene_shift, (size_t)3 * nscenes));
+    HIPC(dalloc(&d.scene_err, nscenes));
+    HIPC(dalloc(&d.scene_cnt, nscenes));
+    c->cap_scenes = nscenes;
+  }
+  HIPC(dalloc(&d.bscene, c->cap_bodies));
+  HIPC(hipMemcpyAsync(d.bscene, bs.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.scene_poff, plane_off, sizeof(uint32_t) * (nscenes + 1), hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.scene_mu, mu, sizeof(double) * nscenes, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.scene_shift, shift.data(), sizeof(double) * 3 * nscenes, hipMemcpyHostToDevice, s));
+  HIPC(hipStreamSynchronize(s));
+  c->nscenes = nscenes;
+  c->scene_planes = most;
+  c->last_ncontacts = 0;
+  return SCX_OK;
+}
+
+int scx_get_scene_stats(scx_ctx* c, uint32_t nscenes, uint32_t* ncontacts, double* error) {
+  if (!c) return SCX_E_ARG;
+  if (c->nscenes == 0 || nscenes != c->nscenes)
+    return fail(c, SCX_E_ARG, "scx_get_scene_stats: the context holds %u scenes", c->nscenes);
+  HIPC(hipSetDevice(c->device));
+  DeviceBuffers& d = c->d;
+  HIPC(launch_scene_stats(c, c->last_ncontacts));
+  if (ncontacts)
+    HIPC(hipMemcpyAsync(ncontacts, d.scene_cnt, sizeof(uint32_t) * nscenes, hipMemcpyDeviceToHost, c->stream));
+  if (error) HIPC(hipMemcpyAsync(error, d.scene_err, sizeof(double) * nscenes, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
   return SCX_OK;
 }
 
@@ -709,6 +844,7 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   // the explicit problem replaces the resident scene; its bodies keep their own indices
   c->has_bodies = false;
   c->n = nbodies;
+  c->nscenes = 0;
   hipStream_t s = c->stream;
   DeviceBuffers& d = c->d;
   d.sid = nullptr;
@@ -756,7 +892,6 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   HIPC(hipMemcpyAsync(d.v0, v0, sizeof(double) * D * nbodies, hipMemcpyHostToDevice, s));
   HIPC(hipMemcpyAsync(d.w0, w0.data(), sizeof(double) * 3 * nbodies, hipMemcpyHostToDevice, s));
   HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
-  uint32_t max_colour = 0;
   if (C > 0) {
     HIPC(hipMemcpyAsync(d.ca, ca.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
     HIPC(hipMemcpyAsync(d.cb, cb.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
@@ -772,7 +907,6 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
     HIPC(launch_colour_sort(c, C));
     HIPC(hipMemcpyAsync(d.host_u32 + 2, d.coloff, sizeof(uint32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
-    for (int b = 0; b < kMaxColours; ++b) max_colour = std::max(max_colour, d.host_u32[3 + b] - d.host_u32[2 + b]);
     HIPC(launch_gather_records(c, C, precision == SCX_PRECISION_FP32_POLISH, 0.0, d.calpha, d.crinit));
   }
   scx_step_params p{};
@@ -783,7 +917,7 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   p.precision = precision;
   p.warm_start = warm ? SCX_WARM_CACHE : SCX_WARM_NONE;
   p.mu = 1.0;  // beta is taken as solved (per-contact mu is in the records)
-  HIPC(launch_solve(c, C, max_colour, &p));
+  HIPC(launch_solve(c, C, d.host_u32 + 2, &p));
   if (C > 0) {
     const uint32_t nc = C;
     (void)nc;

@@ -125,8 +125,10 @@ __global__ void __launch_bounds__(kBlock) k_predict_aabb(uint32_t n, const uint3
                                                          double dt, double gx, double gy, double gz, int integrator,
                                                          double* __restrict__ q1p, float4* __restrict__ tbox,
                                                          double4* __restrict__ tnar, double* __restrict__ tr,
+                                                         const uint32_t* __restrict__ bscene,
+                                                         const double* __restrict__ shift,
                                                          double* __restrict__ partials) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t t = xcd_block() * kBlock + threadIdx.x;
   const bool valid = t < n;
   const double g[3] = {gx, gy, gz};
   double lo[D], hi[D];
@@ -170,6 +172,17 @@ __global__ void __launch_bounds__(kBlock) k_predict_aabb(uint32_t n, const uint3
 #pragma unroll
       for (int a = 0; a < D; ++a) q1p[(size_t)D * i + a] = q1[a];
       tnar[t] = make_double4(q1[0], q1[1], q1[D - 1], ri);
+    }
+    if (bscene) {
+      // ensembles: each scene binned at its own place; the FP64 narrow data stays
+      // unshifted, and rounding is monotone, so the shifted FP32 box still contains the
+      // shifted FP64 box and rejects no pair of the scene
+      const double* sh = shift + 3 * (size_t)bscene[i];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        lo[a] = lo[a] + sh[a];
+        hi[a] = hi[a] + sh[a];
+      }
     }
     store_box32<D>(tbox, t, lo, hi);
   }
@@ -215,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_boxes_to_sweep(uint32_t n, const uin
                                                            const double* __restrict__ amin,
                                                            const double* __restrict__ amax, float4* __restrict__ tbox,
                                                            double4* __restrict__ tnar) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= n) return;
   const uint32_t i = order ? order[t] : t;
   double lo[D], hi[D];
@@ -326,7 +339,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_cells_count(uint32_t n, const float4* __restrict__ tbox,
                                                         const GridParams* __restrict__ gp, uint32_t* __restrict__ ccnt,
                                                         uint32_t* __restrict__ hcnt, StepCounters* __restrict__ ctr) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t t = xcd_block() * kBlock + threadIdx.x;
   const GridParams g = *gp;
   uint32_t cnt = 0;
   if (t < n) {
@@ -361,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) k_entries_scatter(uint32_t n, const fl
                                                             const uint32_t* __restrict__ order_in,
                                                             uint32_t* __restrict__ order_out, uint32_t* __restrict__ ent,
                                                             uint32_t cap, StepCounters* __restrict__ ctr) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= n) return;
   const GridParams g = *gp;
   float bl[3], bh[3];
@@ -419,6 +432,7 @@ struct SweepBodies {
   const float4* box;
   const double4* nar;
   const double* r;
+  const uint32_t* scene;  // ensembles: scene of each body id (pairs only within a scene), else nullptr
 };
 
 // Exact FP64 data of one body: its box as the FP64 test sees it, plus the narrow data.
@@ -498,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(uint32_t n, SweepBodies sb, co
                                                   uint32_t* __restrict__ cnt_off, uint32_t* __restrict__ pa,
                                                   uint32_t* __restrict__ pb, uint32_t cap,
                                                   StepCounters* __restrict__ ctr) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= n) return;
   const GridParams g = *gp;
   const uint32_t i = sb.order[t];
@@ -508,6 +522,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(uint32_t n, SweepBodies sb, co
   cell_range_f<D>(bl, bh, g, lo, hi);
   Exact<D, FILTER> me;
   load_exact<D, FILTER>(sb, t, me);
+  const uint32_t my_scene = sb.scene ? sb.scene[i] : 0u;
   uint32_t base = 0, count = 0;
   if (MODE == 1) base = cnt_off[i];
   for (uint32_t z = lo[2]; z <= hi[2]; ++z)
@@ -535,6 +550,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(uint32_t n, SweepBodies sb, co
           if (sep) continue;
           const uint32_t j = sb.order[tt];
           if (j <= i) continue;
+          if (sb.scene && sb.scene[j] != my_scene) continue;
           Exact<D, FILTER> other;
           load_exact<D, FILTER>(sb, tt, other);
           if (!exact_test<D, FILTER>(me, other)) continue;
@@ -577,6 +593,8 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_plane_masks(uint32_t n, const double* __restrict__ q1,
                                                         const double* __restrict__ r, uint32_t nplanes,
                                                         const double* __restrict__ px, const double* __restrict__ pn,
+                                                        const uint32_t* __restrict__ bscene,
+                                                        const uint32_t* __restrict__ scene_poff,
                                                         unsigned long long* __restrict__ mask) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -585,11 +603,18 @@ __global__ void __launch_bounds__(kBlock) k_plane_masks(uint32_t n, const double
   for (int a = 0; a < D; ++a) x[a] = q1[(size_t)D * i + a];
   const double ri = r[i];
   unsigned long long m = 0;
-  for (uint32_t p = 0; p < nplanes; ++p) {
+  // ensembles: the planes of the body's scene, bits scene-local
+  uint32_t p0 = 0, p1 = nplanes;
+  if (bscene) {
+    const uint32_t sc = bscene[i];
+    p0 = scene_poff[sc];
+    p1 = scene_poff[sc + 1];
+  }
+  for (uint32_t p = p0; p < p1; ++p) {
     double d = pn[D * p] * (x[0] - px[D * p]);
 #pragma unroll
     for (int a = 1; a < D; ++a) d = d + pn[D * p + a] * (x[a] - px[D * p + a]);
-    if (d <= ri) m |= 1ull << p;
+    if (d <= ri) m |= 1ull << (p - p0);
   }
   mask[i] = m;
 }
@@ -603,12 +628,13 @@ hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, c
   const uint32_t nblk = (n + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
   DeviceBuffers& d = c->d;
+  const uint32_t* bscene = c->nscenes ? d.bscene : nullptr;
   if (c->dim == 2)
     k_predict_aabb<2><<<nblk, kBlock, 0, c->stream>>>(n, d.order, q0, v0, m, d.r, dt, g[0], g[1], g[2], integrator,
-                                                       q1p, d.tbox, d.tnar, d.tr, d.red);
+                                                       q1p, d.tbox, d.tnar, d.tr, bscene, d.scene_shift, d.red);
   else
     k_predict_aabb<3><<<nblk, kBlock, 0, c->stream>>>(n, d.order, q0, v0, m, d.r, dt, g[0], g[1], g[2], integrator,
-                                                       q1p, d.tbox, d.tnar, d.tr, d.red);
+                                                       q1p, d.tbox, d.tnar, d.tr, bscene, d.scene_shift, d.red);
   return hipGetLastError();
 }
 
@@ -704,7 +730,7 @@ template <int D, int FILTER>
 static void pairs_pass(scx_ctx* c, uint32_t n, int mode, uint32_t nblk, uint32_t* out_a, uint32_t* out_b,
                        uint32_t cap) {
   DeviceBuffers& d = c->d;
-  const SweepBodies sb{d.order, d.tbox, d.tnar, d.tr};
+  const SweepBodies sb{d.order, d.tbox, d.tnar, d.tr, (FILTER != kFilterNone && c->nscenes) ? d.bscene : nullptr};
   const uint32_t* cstart = d.cells + (c->cap_cells + 1);
   if (mode == 0)
     k_pairs<D, 0, FILTER><<<nblk, kBlock, 0, c->stream>>>(n, sb, d.grid, cstart, d.ent, c->cap_entries, d.pcount,
@@ -737,10 +763,13 @@ hipError_t launch_pairs(scx_ctx* c, uint32_t n, int mode, int filter, uint32_t* 
 hipError_t launch_plane_masks(scx_ctx* c, uint32_t n) {
   const uint32_t nblk = (n + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
+  const uint32_t* bscene = c->nscenes ? c->d.bscene : nullptr;
   if (c->dim == 2)
-    k_plane_masks<2><<<nblk, kBlock, 0, c->stream>>>(n, c->d.q1p, c->d.r, c->nplanes, c->d.px, c->d.pn, c->d.pmask);
+    k_plane_masks<2><<<nblk, kBlock, 0, c->stream>>>(n, c->d.q1p, c->d.r, c->nplanes, c->d.px, c->d.pn, bscene,
+                                                     c->d.scene_poff, c->d.pmask);
   else
-    k_plane_masks<3><<<nblk, kBlock, 0, c->stream>>>(n, c->d.q1p, c->d.r, c->nplanes, c->d.px, c->d.pn, c->d.pmask);
+    k_plane_masks<3><<<nblk, kBlock, 0, c->stream>>>(n, c->d.q1p, c->d.r, c->nplanes, c->d.px, c->d.pn, bscene,
+                                                     c->d.scene_poff, c->d.pmask);
   return hipGetLastError();
 }
 

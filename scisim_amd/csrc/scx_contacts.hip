@@ -66,7 +66,9 @@ struct SetupArgs {
   uint32_t nbb, npl;
   const uint32_t* plane_body;    // plane contact k - nbb -> body (plane-major order)
   const uint32_t* plane_off;     // nplanes + 1 offsets into plane_body
-  uint32_t nplanes;
+  uint32_t nplanes;              // plane buckets (ensembles: planes per scene, scene-local)
+  const uint32_t* bscene;        // ensembles: scene of each body (else nullptr)
+  const uint32_t* scene_poff;    // ensembles: first plane of each scene
   const double* q0;
   const double* v0;
   const double* w0;
@@ -106,6 +108,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_setup(SetupArgs A) {
     b = kNoBody;
     plane = 0;
     while ((uint32_t)(plane + 1) < A.nplanes && A.plane_off[plane + 1] <= kp) ++plane;
+    if (A.bscene) plane += (int32_t)A.scene_poff[A.bscene[a]];
     A.ca[k] = a;
     A.cb[k] = b;
   }
@@ -197,7 +200,9 @@ hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double C
   A.npl = npl;
   A.plane_body = c->d.perm;  // plane contacts' bodies, written by the plane compaction
   A.plane_off = c->d.coloff; // plane offsets, likewise
-  A.nplanes = c->nplanes;
+  A.nplanes = c->nscenes ? c->scene_planes : c->nplanes;
+  A.bscene = c->nscenes ? c->d.bscene : nullptr;
+  A.scene_poff = c->d.scene_poff;
   A.q0 = c->d.q0;
   A.v0 = c->d.v0;
   A.w0 = c->d.w0;
@@ -626,6 +631,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_features(uint32_t C, const u
                                                              const double* __restrict__ r,
                                                              const double* __restrict__ px,
                                                              const double* __restrict__ rprev, double vref,
+                                                             const double* __restrict__ mu_arr,
                                                              float* __restrict__ X, float* __restrict__ S) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= C) return;
@@ -659,7 +665,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_features(uint32_t C, const u
     f[3 + j] = (float)(u / vref);
   }
   f[6] = (float)(wt[k] / wn[k]);
-  f[7] = (float)mu;
+  f[7] = (float)(mu_arr ? mu_arr[k] : mu);
   bool prev = false;
   for (int j = 0; j < D; ++j) {
     const double rp = rprev[(size_t)D * k + j];
@@ -691,6 +697,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_features(uint32_t C, const u
 // Predicted impulses (over s) into the warm start: r = s y, normal clamped at zero and
 // the tangential part projected onto the friction cone |r_T| <= mu r_n.
 __global__ void __launch_bounds__(kBlock) k_apply_prediction(uint32_t C, int dim, double mu,
+                                                             const double* __restrict__ mu_arr,
                                                              const float* __restrict__ Y,
                                                              const float* __restrict__ S,
                                                              double* __restrict__ rinit) {
@@ -704,7 +711,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_prediction(uint32_t C, int dim
     nt += rt[j - 1] * rt[j - 1];
   }
   nt = sqrt(nt);
-  const double lim = mu * rn;
+  const double lim = (mu_arr ? mu_arr[k] : mu) * rn;
   const double sc = (nt > lim && nt > 0.0) ? lim / nt : 1.0;
   rinit[(size_t)dim * k] = rn;
   for (int j = 1; j < dim; ++j) rinit[(size_t)dim * k + j] = rt[j - 1] * sc;
@@ -722,6 +729,22 @@ __global__ void __launch_bounds__(kBlock) k_sample_targets(uint32_t C, int dim, 
   T[(size_t)3 * k + 2] = (dim > 2) ? (float)(beta[(size_t)(dim - 1) * k + 1] / s) : 0.f;
 }
 
+// Friction coefficient of each contact of an ensemble: that of the scene of body a.
+__global__ void __launch_bounds__(kBlock) k_contact_mu(uint32_t C, const uint32_t* __restrict__ ca,
+                                                       const uint32_t* __restrict__ bscene,
+                                                       const double* __restrict__ scene_mu, double* __restrict__ cmu) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  cmu[k] = scene_mu[bscene[ca[k]]];
+}
+
+hipError_t launch_contact_mu(scx_ctx* c, uint32_t C) {
+  const uint32_t nblk = (C + kBlock - 1) / kBlock;
+  if (nblk == 0 || c->nscenes == 0) return hipSuccess;
+  k_contact_mu<<<nblk, kBlock, 0, c->stream>>>(C, c->d.ca, c->d.bscene, c->d.scene_mu, c->d.cmu);
+  return hipGetLastError();
+}
+
 hipError_t launch_contact_features(scx_ctx* c, uint32_t C, double mu, double vref) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
@@ -729,18 +752,21 @@ hipError_t launch_contact_features(scx_ctx* c, uint32_t C, double mu, double vre
   if (c->dim == 2)
     k_contact_features<2><<<nblk, kBlock, 0, c->stream>>>(C, d.ca, d.cb, d.cplane, d.basis, d.cla, d.clb, d.cwn,
                                                            d.cwt, mu, d.cforcing, d.q0, d.v0, d.w0, d.r, d.px,
-                                                           d.crinit, vref, d.mlp_x, d.mlp_s);
+                                                           d.crinit, vref, c->nscenes ? d.cmu : nullptr, d.mlp_x,
+                                                           d.mlp_s);
   else
     k_contact_features<3><<<nblk, kBlock, 0, c->stream>>>(C, d.ca, d.cb, d.cplane, d.basis, d.cla, d.clb, d.cwn,
                                                            d.cwt, mu, d.cforcing, d.q0, d.v0, d.w0, d.r, d.px,
-                                                           d.crinit, vref, d.mlp_x, d.mlp_s);
+                                                           d.crinit, vref, c->nscenes ? d.cmu : nullptr, d.mlp_x,
+                                                           d.mlp_s);
   return hipGetLastError();
 }
 
 hipError_t launch_apply_prediction(scx_ctx* c, uint32_t C, double mu) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
-  k_apply_prediction<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mu, c->d.mlp_y, c->d.mlp_s, c->d.crinit);
+  k_apply_prediction<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mu, c->nscenes ? c->d.cmu : nullptr, c->d.mlp_y,
+                                                     c->d.mlp_s, c->d.crinit);
   return hipGetLastError();
 }
 

@@ -69,12 +69,33 @@ __device__ __forceinline__ double wave_reduce_sum_f64(double v) {
   return v;
 }
 
+// Logical workgroup index for launches whose neighbouring workgroups share data. The
+// dispatcher hands workgroups to the XCDs round-robin (workgroup b to XCD b % 8) and every
+// XCD has its own L2; renumbering gives each XCD one contiguous range of logical indices
+// (a bijection for any grid size), so lines shared by neighbouring tiles are fetched
+// into one L2 instead of up to eight.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t b = blockIdx.x, nb = gridDim.x;
+  const uint32_t q = nb / kXcds, rem = nb % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * q + min(x, rem) + i;
+}
+
+// Waits until this wave's outstanding memory operations (stores included) have completed.
+__device__ __forceinline__ void wait_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Grid-wide barrier for persistent kernels (launched cooperatively, so that every
 // workgroup is resident). Two levels: workgroups arrive on their group's counter
 // (groups of kBarrierGroup, one 128-byte line each); the last of a group arrives on the
 // top counter; the last group bumps the generation word that everyone else polls.
 // Counters return to zero, so the state is reusable without a reset.
-// Fences: the workgroup barrier first completes every wave's stores to the L2; then one
+// Fences: every wave first waits for its own stores to complete (vmcnt 0; a workgroup
+// barrier alone does not wait for them on CDNA), then one
 // wave per workgroup fences at agent scope before arriving (the L2 write-back that
 // publishes the workgroup's stores to the other XCDs) and after leaving (the cache
 // invalidation that drops stale lines for the whole CU). An agent-scope fence costs a
@@ -92,6 +113,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
     __syncthreads();
     return;
   }
+  wait_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
@@ -121,24 +143,34 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
   __syncthreads();
 }
 
-// Workgroups for a persistent pass over `width` items per barrier interval, with
-// `threads` per workgroup: the count minimising (barrier + passes) time, from timings on
-// MI355X. One workgroup: ~0.2 us barrier, ~1.9 us per pass (caches stay warm). Several:
-// ~0.6 us + 0.1 us per workgroup of barrier, ~4.5 us for the first pass after the cache
-// invalidation and ~3.5 us for each further one.
-inline int persistent_blocks(uint32_t width, int threads, int max_blocks) {
+// Time (us) of one barrier interval of a persistent launch over `width` items with `b`
+// workgroups of `threads`, from timings on MI355X. One workgroup: ~0.2 us barrier, ~1.9 us
+// per pass (caches stay warm). Several: ~0.6 us + 0.1 us per workgroup of barrier, ~4.5 us
+// for the first pass after the cache invalidation and ~3.5 us for each further one.
+inline double interval_cost(uint32_t width, int threads, int b) {
+  double passes = (double)((width + (uint64_t)threads * b - 1) / ((uint64_t)threads * b));
+  if (passes < 1.0) passes = 1.0;
+  return (b == 1) ? 0.2 + 1.9 * passes : (0.6 + 0.1 * b) + 4.5 + 3.5 * (passes - 1.0);
+}
+
+// Workgroups (a power of two up to max_blocks) minimising the summed time of barrier
+// intervals over the given widths (one launch runs them all with the same width).
+inline int persistent_blocks(const uint32_t* widths, int nwidths, int threads, int max_blocks) {
   double best = 1e30;
   int pick = 1;
   for (int b = 1; b <= max_blocks; b *= 2) {
-    double passes = (double)((width + (uint64_t)threads * b - 1) / ((uint64_t)threads * b));
-    if (passes < 1.0) passes = 1.0;
-    const double t = (b == 1) ? 0.2 + 1.9 * passes : (0.6 + 0.1 * b) + 4.5 + 3.5 * (passes - 1.0);
+    double t = 0.0;
+    for (int k = 0; k < nwidths; ++k) t += interval_cost(widths[k], threads, b);
     if (t < best) {
       best = t;
       pick = b;
     }
   }
   return pick;
+}
+
+inline int persistent_blocks(uint32_t width, int threads, int max_blocks) {
+  return persistent_blocks(&width, 1, threads, max_blocks);
 }
 
 // Contact priority for the colouring rounds: (fmix32(c * golden + salt) << 32) | c.

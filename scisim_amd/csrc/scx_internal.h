@@ -17,6 +17,8 @@ constexpr int kScanItems = 8;         // items per thread in scans
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr uint32_t kNoBody = 0xFFFFFFFFu;
 constexpr int kMaxColours = 64;
+constexpr uint32_t kMaxScenePlanes = 64;        // static planes of one scene (64-bit plane masks)
+constexpr uint32_t kMaxPlanesTotal = 1u << 24;  // static planes of a context (ensembles)
 
 // Learned warm start: 16-128-128-3 perceptron; FP32 parameter vector layout
 // W1 [16][128], b1 [128], W2 [128][128], b2 [128], W3 [128][3], b3 [3].
@@ -104,6 +106,14 @@ struct DeviceBuffers {
   // planes
   double* px = nullptr;      // dim*P
   double* pn = nullptr;      // dim*P (normalised)
+  // ensemble of independent scenes (scx_set_scenes); unused when nscenes == 0
+  uint32_t* bscene = nullptr;      // N: scene of each body
+  uint32_t* scene_poff = nullptr;  // S + 1: plane ranges
+  double* scene_mu = nullptr;      // S: friction coefficients
+  double* scene_shift = nullptr;   // 3 S: binning shift of each scene (keeps scenes apart in the grid)
+  double* scene_err = nullptr;     // S: per-scene residual (scx_get_scene_stats)
+  uint32_t* scene_cnt = nullptr;   // S: per-scene contacts
+  double* cmu = nullptr;           // per contact friction coefficient (cap_contacts)
   // broad phase
   double* aabb_min = nullptr;  // dim*N
   double* aabb_max = nullptr;
@@ -196,6 +206,8 @@ struct scx_ctx {
   uint32_t cap_bodies = 0, cap_planes = 0, cap_contacts = 0, cap_entries = 0, cap_pairs = 0, cap_cells = 0;
   uint32_t cap_perm = 0;  // perm buffer length (plane compaction and colour order)
   uint32_t n = 0, nplanes = 0;
+  uint32_t nscenes = 0, cap_scenes = 0;
+  uint32_t scene_planes = 0;  // most planes of one scene
   bool has_bodies = false;
   bool order_valid = false;  // d.order holds a permutation of the current n bodies
   uint32_t last_ncontacts = 0, last_npairs = 0, last_nplane = 0, last_ncolours = 0;
@@ -232,11 +244,16 @@ hipError_t launch_plane_masks(scx_ctx* c, uint32_t n);
 hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint32_t n, uint32_t nbuckets,
                               uint32_t* out_idx, uint32_t out_cap, uint32_t* bucket_offsets);
 hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double mu);
+// Ensembles: d.cmu[k] = mu of the scene of contact k (no-op for one scene).
+hipError_t launch_contact_mu(scx_ctx* c, uint32_t C);
+// Ensembles: per-scene contacts and residual of the last solve.
+hipError_t launch_scene_stats(scx_ctx* c, uint32_t C);
 hipError_t launch_colouring(scx_ctx* c, uint32_t C);
 hipError_t launch_colour_sort(scx_ctx* c, uint32_t C);
 hipError_t launch_gather_records(scx_ctx* c, uint32_t C, bool fp32, double mu, const double* mu_arr,
                                  const double* rinit);
-hipError_t launch_solve(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p);
+// col_off: the host copy of the kMaxColours + 1 colour offsets.
+hipError_t launch_solve(scx_ctx* c, uint32_t C, const uint32_t* col_off, const scx_step_params* p);
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p);
 hipError_t launch_impulses(scx_ctx* c, uint32_t C);
 // Impulse cache: keys of this step's active set and the warm start from the previous

@@ -758,15 +758,23 @@ __global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, co
   for (int d = 1; d < dim; ++d) beta[(size_t)(dim - 1) * c + d - 1] = (mu == 0.0) ? 0.0 : r64[3 * (size_t)k + d];
 }
 
-// Persistent launch sized by the widest colour and the barrier/pass trade-off of
-// persistent_blocks (the barrier's cost grows with the workgroups): one workgroup (a
-// workgroup barrier, no device fence) for narrow colours, more as colours widen.
-// Largest dynamic LDS of the single-workgroup solve (gfx950: 160 KB per workgroup).
-constexpr size_t kGSLdsMax = 150 * 1024;
+// Persistent launch sized by the barrier/pass trade-off of persistent_blocks over one
+// sweep's intervals (the barrier's cost grows with the workgroups): one workgroup (a
+// workgroup barrier, no device fence) for small contact sets, more as colours widen.
+// Largest dynamic LDS of the single-workgroup solve: sweeps measured on MI355X ran
+// about three times slower per sweep once the allocation passed 128 KB, so the LDS
+// solve stays below that.
+constexpr size_t kGSLdsMax = 124 * 1024;
 
 template <int D, typename Real>
-static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, uint32_t max_colour) {
-  const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(max_colour, kGSBlock, max_blocks);
+static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint32_t* col_off) {
+  // per sweep: one barrier interval per colour and one for the residual over all contacts
+  uint32_t widths[kMaxColours + 1];
+  int nw = 0;
+  for (int b = 0; b < kMaxColours; ++b)
+    if (col_off[b + 1] > col_off[b]) widths[nw++] = col_off[b + 1] - col_off[b];
+  widths[nw++] = A.C;
+  const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks);
   const int blocks = std::max(1, std::min(max_blocks, want));
   const size_t lds = LdsLayout<D, Real>::bytes(A.C);
   if (blocks == 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap) {
@@ -791,7 +799,7 @@ int gs_max_blocks(scx_ctx* c) {
 }
 
 template <int D>
-static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p) {
+static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, const scx_step_params* p) {
   hipStream_t s = c->stream;
   const bool fp32 = p->precision == SCX_PRECISION_FP32_POLISH;
   const bool warm = p->warm_start != SCX_WARM_NONE;
@@ -807,7 +815,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
                     reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
     A.lmap = c->d.lmap;
-    e = coop_gs<D, float>(c, A, c->gs_blocks, max_colour);
+    e = coop_gs<D, float>(c, A, c->gs_blocks, col_off);
     if (e != hipSuccess) return e;
     timing_mark(c, "gs_fp32");
     const uint32_t nr = (3 * C + kBlock - 1) / kBlock;
@@ -820,11 +828,11 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, uint32_t max_colour, const s
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
   B.lmap = c->d.lmap;
-  return coop_gs<D, double>(c, B, c->gs_blocks, max_colour);
+  return coop_gs<D, double>(c, B, c->gs_blocks, col_off);
 }
 
-hipError_t launch_solve(scx_ctx* c, uint32_t C, uint32_t max_colour, const scx_step_params* p) {
-  return c->dim == 2 ? solve_dim<2>(c, C, max_colour, p) : solve_dim<3>(c, C, max_colour, p);
+hipError_t launch_solve(scx_ctx* c, uint32_t C, const uint32_t* col_off, const scx_step_params* p) {
+  return c->dim == 2 ? solve_dim<2>(c, C, col_off, p) : solve_dim<3>(c, C, col_off, p);
 }
 
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {
@@ -846,6 +854,40 @@ hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {
 hipError_t launch_impulses(scx_ctx* c, uint32_t C) {
   const uint32_t nc = (C + kBlock - 1) / kBlock;
   if (nc) k_impulses_out<<<nc, kBlock, 0, c->stream>>>(C, c->dim, c->d.perm, c->d.c64.r, c->d.calpha, c->d.cbeta, 1.0);
+  return hipGetLastError();
+}
+
+// Per-scene contacts and residual (max ||FB||^2 over the scene's contacts) of the last
+// solve of an ensemble, from the FP64 sweep records and body records it left.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_scene_stats(GSArgs<double> A, const uint32_t* __restrict__ bscene,
+                                                        double* __restrict__ err, uint32_t* __restrict__ cnt) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= A.C) return;
+  const uint32_t sl = A.c.a[k];
+  const uint32_t sc = bscene[A.sid ? A.sid[sl] : sl];
+  const double e = contact_err<D, double>(A, k);
+  // residuals are >= 0, so their bit patterns order like their values
+  atomicMax(reinterpret_cast<unsigned long long*>(err) + sc, (unsigned long long)__double_as_longlong(e));
+  atomicAdd(cnt + sc, 1u);
+}
+
+hipError_t launch_scene_stats(scx_ctx* c, uint32_t C) {
+  DeviceBuffers& d = c->d;
+  hipError_t e = hipMemsetAsync(d.scene_err, 0, sizeof(double) * c->nscenes, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d.scene_cnt, 0, sizeof(uint32_t) * c->nscenes, c->stream);
+  const uint32_t nc = (C + kBlock - 1) / kBlock;
+  if (e != hipSuccess || nc == 0) return e;
+  GSArgs<double> A{};
+  A.c = d.c64;
+  A.body = d.bv64;
+  A.sid = d.sid;
+  A.nbodies = c->n;
+  A.C = C;
+  if (c->dim == 2)
+    k_scene_stats<2><<<nc, kBlock, 0, c->stream>>>(A, d.bscene, d.scene_err, d.scene_cnt);
+  else
+    k_scene_stats<3><<<nc, kBlock, 0, c->stream>>>(A, d.bscene, d.scene_err, d.scene_cnt);
   return hipGetLastError();
 }
 

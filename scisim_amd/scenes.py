@@ -24,7 +24,8 @@ def lib():
         L.scn_rsa3d.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64, _pd, _pd, _pd]
         L.scn_deposit3d.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, _pd, _pd, _pd]
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                    _pd, _pd, _pd]
         _lib = L
     return _lib
 
@@ -95,23 +96,54 @@ def spheres_rsa(n=262144, rmed=0.05, sigma=0.25, base=None, phi=0.3, seed=1, den
             "pn": pn, "base": base, "height": height.value}
 
 
-def spheres_pile(n=262144, rmed=0.05, sigma=0.25, base=None, seed=1, density=1.0, sink=1e-9):
-    """A poured pile at rest: log-normal spheres dropped one by one onto the floor of a
-    box (floor + 4 walls) and rolled downhill into a pocket of their supports (about six
-    contacts per sphere). Each sphere overlaps its supports by `sink` so that its contacts
-    are active from the first step. v = 0."""
+def spheres_pile(n=262144, rmed=0.05, sigma=0.25, base=None, seed=1, density=1.0, sink=1e-9, rmin=None, rmax=None):
+    """A poured pile at rest: spheres dropped one by one onto the floor of a box (floor +
+    4 walls) and rolled downhill into a pocket of their supports (about six contacts per
+    sphere). Radii log-normal (median rmed, sigma), or uniform in [rmin, rmax] when both
+    are given. Each sphere overlaps its supports by `sink` so that its contacts are active
+    from the first step. v = 0."""
+    uniform = rmin is not None and rmax is not None
     if base is None:
         # base so that the pile (volume fraction ~0.58) is roughly as tall as wide
-        vol = n * 4.0 / 3.0 * np.pi * rmed ** 3 * np.exp(4.5 * sigma * sigma)
+        if uniform:
+            r3 = (rmax ** 4 - rmin ** 4) / (4.0 * (rmax - rmin))  # E[r^3] of U[rmin, rmax]
+        else:
+            r3 = rmed ** 3 * np.exp(4.5 * sigma * sigma)
+        vol = n * 4.0 / 3.0 * np.pi * r3
         base = (vol / 0.58) ** (1.0 / 3.0)
     q = np.empty(3 * n)
     r = np.empty(n)
     height = ctypes.c_double(0.0)
-    rc = lib().scn_deposit3d(n, rmed, sigma, rmed * np.exp(-5 * sigma), rmed * np.exp(5 * sigma), base, seed, sink,
-                             _p(q), _p(r), ctypes.byref(height))
+    lo, hi = (rmin, rmax) if uniform else (rmed * np.exp(-5 * sigma), rmed * np.exp(5 * sigma))
+    rc = lib().scn_deposit3d(n, rmed, sigma, lo, hi, 1 if uniform else 0, base, seed, sink, _p(q), _p(r),
+                             ctypes.byref(height))
     if rc != 0:
         raise RuntimeError("deposition failed (%d)" % rc)
     m = density * (4.0 * np.pi * r * r * r / 3.0)
     px, pn = box_planes_3d(base / 2)
     return {"dim": 3, "q": q, "v": np.zeros(3 * n), "r": r, "m": m, "inertia": 2.0 * m * r * r / 5.0, "px": px,
             "pn": pn, "base": base, "height": height.value}
+
+
+def ensemble_piles(nscenes=512, nmin=500, nmax=20000, rmin=0.03, rmax=0.07, mu_lo=0.1, mu_hi=0.8, seed=1,
+                   workers=16):
+    """Independent poured piles for an ensemble: body counts log-uniform in [nmin, nmax],
+    radii uniform in [rmin, rmax], friction coefficients uniform in [mu_lo, mu_hi], all
+    seeded. Every pile sits in its own box at the same coordinates (the library keeps
+    scenes apart). Returns the concatenated scene (q, v, r, m, inertia, px, pn), the body
+    and plane offsets, mu per scene and the body counts."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(nmin), np.log(nmax), nscenes)).round().astype(np.int64)
+    mus = rng.uniform(mu_lo, mu_hi, nscenes)
+    seeds = rng.integers(1, 2 ** 62, nscenes)
+    with ThreadPoolExecutor(max_workers=workers) as ex:  # the generator releases the GIL
+        piles = list(ex.map(lambda k: spheres_pile(n=int(counts[k]), rmin=rmin, rmax=rmax, seed=int(seeds[k])),
+                            range(nscenes)))
+    cat = {k: np.concatenate([p[k] for p in piles]) for k in ("q", "v", "r", "m", "inertia")}
+    cat["px"] = np.concatenate([p["px"] for p in piles])
+    cat["pn"] = np.concatenate([p["pn"] for p in piles])
+    cat["dim"] = 3
+    body_off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    plane_off = np.arange(nscenes + 1, dtype=np.uint32) * 5
+    return cat, body_off, plane_off, mus, counts

@@ -33,7 +33,7 @@ def test_invalid_config_is_rejected_on_the_host():
     h = ctypes.c_void_p()
     for cfg in (scx.ScxConfig(dim=4, device=0, max_bodies=10, max_planes=4),
                 scx.ScxConfig(dim=2, device=0, max_bodies=0, max_planes=4),
-                scx.ScxConfig(dim=3, device=0, max_bodies=10, max_planes=65)):
+                scx.ScxConfig(dim=3, device=0, max_bodies=10, max_planes=(1 << 24) + 1)):
         assert lib.scx_create(ctypes.byref(cfg), ctypes.byref(h)) == scx.SCX_E_ARG
         assert not h
     assert lib.scx_create(None, ctypes.byref(h)) == scx.SCX_E_ARG
