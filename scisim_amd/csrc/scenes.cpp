@@ -241,80 +241,84 @@ int scn_deposit3d(uint32_t n, double rmed, double sigma, double rclamp_lo, doubl
   for (uint32_t i = 0; i < n; ++i) {
     const double ri = radii[i];
     double p[3];
-    p[0] = -base / 2 + ri + (base - 2 * ri) * rng.uniform();
-    p[2] = -base / 2 + ri + (base - 2 * ri) * rng.uniform();
-    // vertical drop: the highest first contact along the line, or the floor
-    double y = ri;
-    {
-      const int cx0 = grid.coord(p[0] - ri - rmax, 0), cx1 = grid.coord(p[0] + ri + rmax, 0);
-      const int cz0 = grid.coord(p[2] - ri - rmax, 2), cz1 = grid.coord(p[2] + ri + rmax, 2);
-      for (int cz = cz0; cz <= cz1; ++cz)
-        for (int cx = cx0; cx <= cx1; ++cx) {
-          const size_t col = (size_t)cz * ncx + cx;
-          if (colmax[col] + ri + rmax <= y) continue;
-          for (uint32_t j : cols[col]) {
-            const double dx = p[0] - q[3 * j], dz = p[2] - q[3 * j + 2], R = ri + r[j];
-            const double h2 = dx * dx + dz * dz;
-            if (h2 < R * R) y = std::max(y, q[3 * j + 1] + std::sqrt(R * R - h2));
-          }
-        }
-    }
-    p[1] = y;
-    // roll downhill: trial moves of `step` straight down, projected out of the
-    // candidates; candidates are gathered with margin ri around g and refreshed when
-    // the sphere has moved ri/2 from g (a trial point then stays within ri of g)
-    double g[3] = {p[0], p[1], p[2]};
-    gather(g, ri, ri);
-    double step = 0.5 * ri;
-    // at most 400 trial moves: a sphere creeping along a shallow groove stops early,
-    // supported but short of its pocket (the first simulated steps finish the job)
-    for (int trial = 0; trial < 400 && step > 1e-6 * ri; ++trial) {
-      const double mx = p[0] - g[0], my = p[1] - g[1], mz = p[2] - g[2];
-      if (mx * mx + my * my + mz * mz > 0.25 * ri * ri) {
-        g[0] = p[0];
-        g[1] = p[1];
-        g[2] = p[2];
-        gather(g, ri, ri);
-      }
-      double t[3] = {p[0], p[1] - step, p[2]};
-      bool ok = false;
-      for (int pass = 0; pass < 16; ++pass) {
-        double worst = 0.0;
-        t[1] = std::max(t[1], ri);
-        for (int a = 0; a < 3; a += 2) t[a] = std::min(base / 2 - ri, std::max(-base / 2 + ri, t[a]));
-        for (uint32_t j : cand) {
-          const double dx = t[0] - q[3 * j], dy = t[1] - q[3 * j + 1], dz = t[2] - q[3 * j + 2];
-          const double R = ri + r[j];
-          const double d2 = dx * dx + dy * dy + dz * dz;
-          if (d2 < R * R) {
-            const double d = std::sqrt(d2);
-            worst = std::max(worst, R - d);
-            if (d > 0) {
-              const double sc = R / d;
-              t[0] = q[3 * j] + dx * sc;
-              t[1] = q[3 * j + 1] + dy * sc;
-              t[2] = q[3 * j + 2] + dz * sc;
-            } else {
-              t[1] = q[3 * j + 1] + R;
+    for (int attempt = 0;; ++attempt) {
+      p[0] = -base / 2 + ri + (base - 2 * ri) * rng.uniform();
+      p[2] = -base / 2 + ri + (base - 2 * ri) * rng.uniform();
+      // vertical drop: the highest first contact along the line, or the floor
+      double y = ri;
+      {
+        const int cx0 = grid.coord(p[0] - ri - rmax, 0), cx1 = grid.coord(p[0] + ri + rmax, 0);
+        const int cz0 = grid.coord(p[2] - ri - rmax, 2), cz1 = grid.coord(p[2] + ri + rmax, 2);
+        for (int cz = cz0; cz <= cz1; ++cz)
+          for (int cx = cx0; cx <= cx1; ++cx) {
+            const size_t col = (size_t)cz * ncx + cx;
+            if (colmax[col] + ri + rmax <= y) continue;
+            for (uint32_t j : cols[col]) {
+              const double dx = p[0] - q[3 * j], dz = p[2] - q[3 * j + 2], R = ri + r[j];
+              const double h2 = dx * dx + dz * dz;
+              if (h2 < R * R) y = std::max(y, q[3 * j + 1] + std::sqrt(R * R - h2));
             }
           }
+      }
+      p[1] = y;
+      // roll downhill: trial moves of `step` straight down, projected out of the
+      // candidates; candidates are gathered with margin ri around g and refreshed when
+      // the sphere has moved ri/2 from g (a trial point then stays within ri of g)
+      double g[3] = {p[0], p[1], p[2]};
+      gather(g, ri, ri);
+      double step = 0.5 * ri;
+      // at most 400 trial moves; a sphere still creeping along a shallow groove after them
+      // (supported, but not in a pocket) is dropped again elsewhere
+      int trial = 0;
+      for (trial = 0; trial < 400 && step > 1e-6 * ri; ++trial) {
+        const double mx = p[0] - g[0], my = p[1] - g[1], mz = p[2] - g[2];
+        if (mx * mx + my * my + mz * mz > 0.25 * ri * ri) {
+          g[0] = p[0];
+          g[1] = p[1];
+          g[2] = p[2];
+          gather(g, ri, ri);
         }
-        if (worst <= 1e-12 * ri && t[1] >= ri && std::fabs(t[0]) <= base / 2 - ri && std::fabs(t[2]) <= base / 2 - ri) {
-          ok = true;
-          break;
+        double t[3] = {p[0], p[1] - step, p[2]};
+        bool ok = false;
+        for (int pass = 0; pass < 16; ++pass) {
+          double worst = 0.0;
+          t[1] = std::max(t[1], ri);
+          for (int a = 0; a < 3; a += 2) t[a] = std::min(base / 2 - ri, std::max(-base / 2 + ri, t[a]));
+          for (uint32_t j : cand) {
+            const double dx = t[0] - q[3 * j], dy = t[1] - q[3 * j + 1], dz = t[2] - q[3 * j + 2];
+            const double R = ri + r[j];
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < R * R) {
+              const double d = std::sqrt(d2);
+              worst = std::max(worst, R - d);
+              if (d > 0) {
+                const double sc = R / d;
+                t[0] = q[3 * j] + dx * sc;
+                t[1] = q[3 * j + 1] + dy * sc;
+                t[2] = q[3 * j + 2] + dz * sc;
+              } else {
+                t[1] = q[3 * j + 1] + R;
+              }
+            }
+          }
+          if (worst <= 1e-12 * ri && t[1] >= ri && std::fabs(t[0]) <= base / 2 - ri && std::fabs(t[2]) <= base / 2 - ri) {
+            ok = true;
+            break;
+          }
+        }
+        const double dmx = t[0] - g[0], dmy = t[1] - g[1], dmz = t[2] - g[2];
+        // keep moves that descend by a fair part of the step; shallow ones halve it (a
+        // sphere sliding down a nearly level groove would otherwise creep at full step)
+        if (ok && t[1] < p[1] - 0.05 * step && dmx * dmx + dmy * dmy + dmz * dmz < ri * ri) {
+          p[0] = t[0];
+          p[1] = t[1];
+          p[2] = t[2];
+          step = std::min(0.5 * ri, 1.5 * step);
+        } else {
+          step *= 0.5;
         }
       }
-      const double dmx = t[0] - g[0], dmy = t[1] - g[1], dmz = t[2] - g[2];
-      // keep moves that descend by a fair part of the step; shallow ones halve it (a
-      // sphere sliding down a nearly level groove would otherwise creep at full step)
-      if (ok && t[1] < p[1] - 0.05 * step && dmx * dmx + dmy * dmy + dmz * dmz < ri * ri) {
-        p[0] = t[0];
-        p[1] = t[1];
-        p[2] = t[2];
-        step = std::min(0.5 * ri, 1.5 * step);
-      } else {
-        step *= 0.5;
-      }
+      if (trial < 400 || attempt >= 63) break;  // in its pocket (no lower position within reach)
     }
     p[1] -= sink;
     q[3 * i] = p[0];

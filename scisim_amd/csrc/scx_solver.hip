@@ -185,8 +185,10 @@ struct GSArgs {
   uint32_t* lmap;           // per body slot: owning endpoint in the LDS solve (all ones when idle)
 };
 
+// One contact of a colour pass. copy_best: first store the contact's impulse as it was
+// before this sweep into rbest (the previous sweep's iterate was the best one so far).
 template <int D, typename Real>
-__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init) {
+__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init, bool copy_best = false) {
   using V = typename Vec4<Real>::type;
   Rec<D, Real> R;
   load_rec<D, Real>(A.c, k, R);
@@ -202,6 +204,12 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
   }
   Real* rp = A.c.r + 3 * (size_t)k;
   const Real r[3] = {rp[0], rp[1], rp[2]};
+  if (copy_best) {
+    Real* rb = A.rbest + 3 * (size_t)k;
+    rb[0] = r[0];
+    rb[1] = r[1];
+    rb[2] = r[2];
+  }
   Real dr[3];
   if (init) {
     dr[0] = r[0];
@@ -344,57 +352,50 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
 
   double err = eval_residual<D, Real>(A, gtid, gsize);
   double best = err;
-  if (track_best) {
-    for (uint32_t k = gtid; k < A.C; k += gsize)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
-    grid_barrier(A.bar);
-  }
-  // FP32 phase stagnation: leave when 256 sweeps lowered the best residual by less than
-  // 1% (FP32 has reached its rounding floor; FP64 continues from there)
+  // Best iterate (So-bogus returns it when the sweep limit is reached): after an
+  // evaluation that lowers the best residual, the next sweep stores every contact's
+  // impulse into rbest as it reads it, before updating it; while that copy is pending the
+  // current iterate is the best one.
+  bool copy_pending = track_best;
+  // FP32 phase stagnation: leave when 512 sweeps lowered the best residual by less than
+  // 0.1% (FP32 has reached its rounding floor; FP64 continues from there)
   double ref = best;
   uint32_t ref_it = 0;
   uint32_t it = 0;
-  bool stagnated = false;
   if (!(err < A.tol)) {
     for (it = 1; it <= max_iters; ++it) {
+      const bool copy = copy_pending;
+      copy_pending = false;
       for (uint32_t col = 0; col < ncol; ++col) {
-        for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false);
+        for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false, copy);
         grid_barrier(A.bar);
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
         err = eval_residual<D, Real>(A, gtid, gsize);
         if (err < best) {
           best = err;
-          if (track_best) {
-            for (uint32_t k = gtid; k < A.C; k += gsize)
-#pragma unroll
-              for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
-            grid_barrier(A.bar);
-          }
+          copy_pending = track_best;
         }
         if (err < A.tol) break;
-        if (A.phase == 1 && it - ref_it >= 256) {
-          if (best > 0.99 * ref) {
-            stagnated = true;
-            break;
-          }
+        if (A.phase == 1 && it - ref_it >= 512) {
+          if (best > 0.999 * ref) break;
           ref = best;
           ref_it = it;
         }
       }
     }
-    if (it > max_iters && track_best) {
-      // not converged: return the best iterate and its velocities (So-bogus keeps x_best)
-      for (uint32_t k = gtid; k < A.C; k += gsize)
+    if (it > max_iters && track_best && !copy_pending) {
+      // not converged and the current iterate is not the best: return the best iterate
+      // and its velocities (So-bogus keeps x_best)
+      for (uint32_t col = 0; col < ncol; ++col)
+        for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) A.c.r[3 * (size_t)k + d] = A.rbest[3 * (size_t)k + d];
+          for (int d = 0; d < 3; ++d) A.c.r[3 * (size_t)k + d] = A.rbest[3 * (size_t)k + d];
       grid_barrier(A.bar);
       apply_impulses<D, Real>(A, off, ncol, gtid, gsize, true);
     }
     if (it > max_iters) it = max_iters;
   }
-  (void)stagnated;
   if (gtid == 0) {
     if (A.phase == 1) {
       A.ctr->sweeps_fp32 = it;
@@ -641,8 +642,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
               for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = sr[3 * (size_t)k + d];
         }
         if (err < A.tol) break;
-        if (A.phase == 1 && it - ref_it >= 256) {
-          if (best > 0.99 * ref) break;
+        if (A.phase == 1 && it - ref_it >= 512) {
+          if (best > 0.999 * ref) break;
           ref = best;
           ref_it = it;
         }
