@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2", "cfg3", "cfg4"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--settle", type=int, default=None,
-                    help="untimed settling steps of the piles (default: cfg2, cfg3 300; cfg4 60)")
+                    help="untimed settling steps of the piles (default: cfg2, cfg3 300; cfg4 200)")
     ap.add_argument("--collect", type=int, default=None,
                     help="steps whose samples train the warm start (default: cfg3 40, cfg4 10)")
     ap.add_argument("--warm", default="none", choices=["none", "cache", "normal", "transport"],
@@ -222,7 +222,7 @@ def main():
     it = 1
     pile = args.config in ("cfg2", "cfg3", "cfg4")
     if args.settle is None:
-        args.settle = 60 if args.config == "cfg4" else 300
+        args.settle = 200 if args.config == "cfg4" else 300
     if args.collect is None:
         args.collect = 10 if args.config == "cfg4" else 40
     if pile and args.settle > 0:

@@ -62,7 +62,9 @@ extern "C" {
                                 counterpart in the reference (the solve still runs to tolerance) */
 
 /* Parameters of the warm-start perceptron (FP32): W1 [16][128], b1 [128], W2 [128][128],
- * b2 [128], W3 [128][3], b3 [3], row-major, y = W3^T relu(W2^T relu(W1^T x + b1) + b2) + b3. */
+ * b2 [128], W3 [128][3], b3 [3], row-major, y = W3^T relu(W2^T relu(W1^T x + b1) + b2) + b3.
+ * The prediction is y + x[8:11]: the network learns a correction to the transported cached
+ * impulse (features 8-10), so a zero network reproduces SCX_WARM_TRANSPORT. */
 #define SCX_MLP_PARAMS 19075
 
 typedef struct scx_ctx scx_ctx;
@@ -200,8 +202,9 @@ int scx_get_warm_model(scx_ctx* ctx, float* params, uint32_t n);
 /* The last step's samples (that step must have used SCX_WARM_TRANSPORT or SCX_WARM_LEARNED):
  * features 16 and targets 3 per contact, in active-set order. */
 int scx_get_samples(scx_ctx* ctx, uint32_t cap, float* features, float* targets, uint32_t* count);
-/* Mean-squared-error loss and its gradient (SCX_MLP_PARAMS, the layout above) over n
- * samples with the current model, from the device's BF16 forward and backward kernels. */
+/* Mean-squared-error loss of the prediction, mean of (y + x[8:11] - t)^2, and its gradient
+ * (SCX_MLP_PARAMS, the layout above) over n samples with the current model, from the
+ * device's BF16 forward and backward kernels. */
 int scx_warm_model_gradients(scx_ctx* ctx, uint32_t n, const float* features, const float* targets, float* grads,
                              double* loss);
 /* Adam training of the current model on the device: `epochs` passes over the n samples in

@@ -63,13 +63,22 @@ def forward(p, X, emulate_bf16=False):
     return x, h1, h2, y
 
 
+BASE = 8  # the network predicts a correction to features 8-10 (the transported cached impulse)
+
+
+def predict(p, X, emulate_bf16=False):
+    """Predicted (scaled) impulses: network output plus features 8-10."""
+    return forward(p, X, emulate_bf16)[3] + np.asarray(X, dtype=np.float64)[:, BASE:BASE + OUT]
+
+
 def loss_and_grad(p, X, T, emulate_bf16=False):
-    """Mean over samples and outputs of (y - t)^2, and its gradient (parameter layout)."""
+    """Mean over samples and outputs of (y + x[8:11] - t)^2, and its gradient (parameter
+    layout)."""
     r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     P = unpack(p)
     x, h1, h2, y = forward(p, X, emulate_bf16)
     n = X.shape[0]
-    d = y - np.asarray(T, dtype=np.float64)
+    d = y + np.asarray(X, dtype=np.float64)[:, BASE:BASE + OUT] - np.asarray(T, dtype=np.float64)
     loss = float((d * d).sum() / (n * OUT))
     gy = r(2.0 * d / (n * OUT))
     g2 = r((gy @ r(P["W3"]).T) * (h2 > 0))

@@ -129,7 +129,12 @@ __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ gw, c
   __syncthreads();
 }
 
-// Inference: Y [n][3] = MLP(X [n][16]). One wave per 16-sample tile, grid-stride.
+// The perceptron predicts a correction to the transported cached impulse: the prediction
+// of a sample is MLP(x) + x[kMlpBase .. kMlpBase + 2] (features 8-10), so an untrained or
+// zero network reproduces the transported-cache warm start.
+constexpr int kMlpBase = 8;
+
+// Inference: Y [n][3] = MLP(X [n][16]) + X[:, 8:11]. One wave per 16-sample tile, grid-stride.
 __global__ void __launch_bounds__(kBlock) k_mlp_infer(uint32_t n, const float* __restrict__ X,
                                                       const uint16_t* __restrict__ gw, const float* __restrict__ gb,
                                                       float* __restrict__ Y) {
@@ -147,13 +152,13 @@ __global__ void __launch_bounds__(kBlock) k_mlp_infer(uint32_t n, const float* _
     // outputs 0..2 of sample lane are in lanes 0..15, elements 0..2
     if (lane < 16 && s < n)
 #pragma unroll
-      for (int o = 0; o < kMlpOut; ++o) Y[(size_t)s * kMlpOut + o] = T.y[o];
+      for (int o = 0; o < kMlpOut; ++o) Y[(size_t)s * kMlpOut + o] = T.y[o] + X[(size_t)s * kMlpIn + kMlpBase + o];
   }
 }
 
 // Training, pass 1: forward and backward of each 16-sample tile of a batch (samples
 // idx[b0 .. b0 + nb) of the pool, or b0 + i when idx is null). Loss = mean over samples
-// and outputs of (y - t)^2. Stores per tile, BF16, unit-major with the 16 samples
+// and outputs of (y + x[8:11] - t)^2. Stores per tile, BF16, unit-major with the 16 samples
 // contiguous: H1T, H2T [tile][128][16], G1T, G2T (gradients w.r.t. the pre-activations)
 // [tile][128][16], GY [tile][16][16] (dLoss/dy, outputs padded to 16), XT [tile][16][16]
 // (features), and the block's sum of squared errors.
@@ -194,7 +199,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_fwdbwd(uint32_t nb, const uint32
     for (int e = 0; e < 4; ++e) {
       const int o = r + e;
       if (valid && o < kMlpOut) {
-        const float d = T.y[e] - Tgt[(size_t)s * kMlpOut + o];
+        const float d = (T.y[e] + X[(size_t)s * kMlpIn + kMlpBase + o]) - Tgt[(size_t)s * kMlpOut + o];
         sse += d * d;
         gy[e] = (short)f2bf(loss_scale * 2.f * d);
       }

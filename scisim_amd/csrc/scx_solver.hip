@@ -120,14 +120,15 @@ __device__ __forceinline__ void local_solve(const Rec<D, Real>& R, const Real u[
   }
 }
 
-// FB residual of one contact (De Saxce change of variable), ||phi||^2, in FP64.
-template <int D>
-__device__ __forceinline__ double fb_err(double mu, const double* r, const double* u) {
-  double uT2 = 0.0;
+// FB residual of one contact (De Saxce change of variable), ||phi||^2, in precision T
+// (FP64 for every decision on the tolerance; FP32 only for the estimate of the FP32 phase).
+template <int D, typename T = double>
+__device__ __forceinline__ T fb_err(T mu, const T* r, const T* u) {
+  T uT2 = (T)0;
 #pragma unroll
   for (int a = 1; a < D; ++a) uT2 += u[a] * u[a];
-  const double un_hat = u[0] + mu * sqrt(uT2);
-  double xh[3], yh[3];
+  const T un_hat = u[0] + mu * sqrt(uT2);
+  T xh[3], yh[3];
   xh[0] = mu * r[0];
   yh[0] = un_hat;
 #pragma unroll
@@ -135,24 +136,24 @@ __device__ __forceinline__ double fb_err(double mu, const double* r, const doubl
     xh[a] = r[a];
     yh[a] = mu * u[a];
   }
-  double z0 = 0.0;
+  T z0 = (T)0;
 #pragma unroll
   for (int a = 0; a < D; ++a) z0 += xh[a] * xh[a] + yh[a] * yh[a];
-  double z1[2] = {0.0, 0.0}, nz1 = 0.0;
+  T z1[2] = {(T)0, (T)0}, nz1 = (T)0;
 #pragma unroll
   for (int a = 1; a < D; ++a) {
-    z1[a - 1] = 2.0 * (xh[0] * xh[a] + yh[0] * yh[a]);
+    z1[a - 1] = (T)2 * (xh[0] * xh[a] + yh[0] * yh[a]);
     nz1 += z1[a - 1] * z1[a - 1];
   }
   nz1 = sqrt(nz1);
-  const double s1 = sqrt(z0 + nz1);
-  const double s2 = sqrt(fmax(0.0, z0 - nz1));
-  double e = xh[0] + yh[0] - 0.5 * (s1 + s2);
+  const T s1 = sqrt(z0 + nz1);
+  const T s2 = sqrt(fmax((T)0, z0 - nz1));
+  T e = xh[0] + yh[0] - (T)0.5 * (s1 + s2);
   e = e * e;
 #pragma unroll
   for (int a = 1; a < D; ++a) {
-    const double dir = nz1 > 0.0 ? z1[a - 1] / nz1 : 0.0;
-    const double p = xh[a] + yh[a] - 0.5 * (s1 - s2) * dir;
+    const T dir = nz1 > (T)0 ? z1[a - 1] / nz1 : (T)0;
+    const T p = xh[a] + yh[a] - (T)0.5 * (s1 - s2) * dir;
     e += p * p;
   }
   return e;
@@ -258,8 +259,8 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
   }
 }
 
-template <int D, typename Real>
-__device__ __forceinline__ double contact_err(const GSArgs<Real>& A, uint32_t k) {
+template <int D, typename Real, typename T = double>
+__device__ __forceinline__ T contact_err(const GSArgs<Real>& A, uint32_t k) {
   using V = typename Vec4<Real>::type;
   Rec<D, Real> R;
   load_rec<D, Real>(A.c, k, R);
@@ -271,11 +272,11 @@ __device__ __forceinline__ double contact_err(const GSArgs<Real>& A, uint32_t k)
   const V wa = (D == 3) ? bw[ia] : z;
   const V vb = (R.b != kNoBody) ? bv[ib] : z;
   const V wb = (D == 3 && R.b != kNoBody) ? bw[ib] : z;
-  double u[3];
-  rel_vel<D, double, Real>(R, va, wa, vb, wb, u);
+  T u[3];
+  rel_vel<D, T, Real>(R, va, wa, vb, wb, u);
   const Real* rp = A.c.r + 3 * (size_t)k;
-  const double r[3] = {(double)rp[0], (double)rp[1], (double)rp[2]};
-  return fb_err<D>((double)R.mu, r, u);
+  const T r[3] = {(T)rp[0], (T)rp[1], (T)rp[2]};
+  return fb_err<D, T>((T)R.mu, r, u);
 }
 
 // Grid-wide max of the residual; identical in every thread after the barrier.
@@ -283,7 +284,10 @@ template <int D, typename Real>
 __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize) {
   __shared__ double s_red[kGSBlock / 64];
   double m = 0.0;
-  for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real>(A, k));
+  if (A.phase == 1)  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
+    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float>(A, k));
+  else
+    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real>(A, k));
   m = wave_reduce_max_f64(m);
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -362,7 +366,10 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   double ref = best;
   uint32_t ref_it = 0;
   uint32_t it = 0;
-  if (!(err < A.tol)) {
+  // the FP32 phase stops on an FP32 estimate with a factor-2 margin; the FP64 phase
+  // that follows decides on the tolerance
+  const double stop = (A.phase == 1) ? 0.5 * A.tol : A.tol;
+  if (!(err < stop)) {
     for (it = 1; it <= max_iters; ++it) {
       const bool copy = copy_pending;
       copy_pending = false;
@@ -376,7 +383,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
           best = err;
           copy_pending = track_best;
         }
-        if (err < A.tol) break;
+        if (err < stop) break;
         if (A.phase == 1 && it - ref_it >= 512) {
           if (best > 0.999 * ref) break;
           ref = best;
@@ -575,10 +582,18 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
       const V wa = (D == 3) ? sb[ia + 1] : z;
       const V vb = (R.b != kNoBody) ? sb[ib] : z;
       const V wb = (D == 3 && R.b != kNoBody) ? sb[ib + 1] : z;
-      double u[3];
-      rel_vel<D, double, Real>(R, va, wa, vb, wb, u);
-      const double r[3] = {(double)sr[3 * (size_t)k], (double)sr[3 * (size_t)k + 1], (double)sr[3 * (size_t)k + 2]};
-      m = fmax(m, fb_err<D>((double)R.mu, r, u));
+      if (A.phase == 1) {  // FP32 phase: an FP32 estimate
+        float u[3];
+        rel_vel<D, float, Real>(R, va, wa, vb, wb, u);
+        const float r[3] = {(float)sr[3 * (size_t)k], (float)sr[3 * (size_t)k + 1], (float)sr[3 * (size_t)k + 2]};
+        m = fmax(m, (double)fb_err<D, float>((float)R.mu, r, u));
+      } else {
+        double u[3];
+        rel_vel<D, double, Real>(R, va, wa, vb, wb, u);
+        const double r[3] = {(double)sr[3 * (size_t)k], (double)sr[3 * (size_t)k + 1],
+                             (double)sr[3 * (size_t)k + 2]};
+        m = fmax(m, fb_err<D>((double)R.mu, r, u));
+      }
     }
     m = wave_reduce_max_f64(m);
     if ((tid & 63) == 0) s_red[tid >> 6] = m;
@@ -627,7 +642,10 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
       for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = sr[3 * (size_t)k + d];
   double ref = best;
   uint32_t ref_it = 0, it = 0;
-  if (!(err < A.tol)) {
+  // the FP32 phase stops on an FP32 estimate with a factor-2 margin; the FP64 phase
+  // that follows decides on the tolerance
+  const double stop = (A.phase == 1) ? 0.5 * A.tol : A.tol;
+  if (!(err < stop)) {
     for (it = 1; it <= max_iters; ++it) {
       for (uint32_t col = 0; col < ncol; ++col) {
         for (uint32_t k = off[col] + tid; k < off[col + 1]; k += kGSBlock) update(k, false);
@@ -641,7 +659,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
             for (uint32_t k = tid; k < C; k += kGSBlock)
               for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = sr[3 * (size_t)k + d];
         }
-        if (err < A.tol) break;
+        if (err < stop) break;
         if (A.phase == 1 && it - ref_it >= 512) {
           if (best > 0.999 * ref) break;
           ref = best;

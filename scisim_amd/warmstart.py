@@ -12,11 +12,15 @@ NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 
 
 def init_params(seed=0):
-    """He-uniform weights (limit sqrt(6 / fan_in)), zero biases."""
+    """He-uniform hidden weights (limit sqrt(6 / fan_in)), zero output weights and biases:
+    the prediction starts as the transported cached impulse (the network adds a learned
+    correction to it)."""
     rng = np.random.default_rng(seed)
     parts = []
     for name, shape in SHAPES:
-        if name.startswith("W"):
+        if name == "W3":
+            parts.append(np.zeros(int(np.prod(shape))))
+        elif name.startswith("W"):
             lim = np.sqrt(6.0 / shape[0])
             parts.append(rng.uniform(-lim, lim, size=shape).ravel())
         else:

@@ -48,6 +48,19 @@ def test_device_gradients_match_oracle(ctx, n):
     assert g_dev.astype(np.float64) @ g_ex >= 0.98 * np.linalg.norm(g_dev) * np.linalg.norm(g_ex)
 
 
+def test_zero_network_predicts_the_transported_impulse(ctx):
+    """With the output layer zero the device prediction is features 8-10 exactly: zero loss
+    and zero gradient against those targets."""
+    rng = np.random.default_rng(2)
+    P = mlp.unpack(mlp.init(seed=6))
+    P["W3"][:] = 0.0
+    P["b3"][:] = 0.0
+    ctx.set_warm_model(mlp.pack(P).astype(np.float32))
+    X = rng.normal(size=(500, 16)).astype(np.float32)
+    g, loss = ctx.warm_model_gradients(X, X[:, 8:11].copy())
+    assert loss == 0.0 and np.all(g == 0.0)
+
+
 def test_device_training_fits_a_smooth_map(ctx):
     """Adam on the device drives the loss of a learnable map well below its start."""
     rng = np.random.default_rng(11)

@@ -46,7 +46,7 @@ def test_gradient_matches_central_differences():
         fd = (mlp.loss_and_grad(p + e, X, T)[0] - mlp.loss_and_grad(p - e, X, T)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
     # zero error gives zero gradient
-    _, _, _, y = mlp.forward(p, X)
+    y = mlp.predict(p, X)  # network output + the transported impulse of features 8-10
     l0, g0 = mlp.loss_and_grad(p, X, y)
     assert l0 == 0.0 and np.all(g0 == 0.0)
 
@@ -60,3 +60,18 @@ def test_bf16_emulation_stays_close_to_exact():
     l_bf, g_bf = mlp.loss_and_grad(p, X, T, emulate_bf16=True)
     assert abs(l_bf - l_ex) <= 2e-2 * l_ex
     assert np.linalg.norm(g_bf - g_ex) <= 5e-2 * np.linalg.norm(g_ex)
+
+
+def test_zero_output_layer_reproduces_the_transported_impulse():
+    """The prediction is the network output plus features 8-10 (the transported cached
+    impulse): with the output layer zero it is exactly those features."""
+    rng = np.random.default_rng(9)
+    p = mlp.init(seed=4)
+    P = mlp.unpack(p)
+    P["W3"][:] = 0.0
+    P["b3"][:] = 0.0
+    p = mlp.pack(P)
+    X = rng.normal(size=(32, 16))
+    assert np.array_equal(mlp.predict(p, X), X[:, 8:11])
+    loss, g = mlp.loss_and_grad(p, X, X[:, 8:11])
+    assert loss == 0.0 and np.all(g == 0.0)
