@@ -168,7 +168,8 @@ def cpu_baseline(s, state, prm, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: cfg1 100, the workload's step count; the 3D piles 20)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2", "cfg3", "cfg4"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
@@ -188,6 +189,8 @@ def main():
                     help="only run 5 plain device copies of BYTES (the reference kernel that scales the HBM "
                          "byte counters of rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.config == "cfg1" else 20
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (requested --gpus %d); multi-GPU scaling is not part of "
               "it" % args.gpus, file=sys.stderr)

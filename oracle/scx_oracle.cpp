@@ -940,8 +940,11 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
       }
     }
   }
-  ctx->cache_bb.clear();
-  ctx->cache_pl.clear();
+  // a step without caching leaves the cache untouched (the map only writes it when caching)
+  if (caching) {
+    ctx->cache_bb.clear();
+    ctx->cache_pl.clear();
+  }
   // order
   std::vector<int> order(C);
   for (int i = 0; i < C; ++i) order[i] = i;

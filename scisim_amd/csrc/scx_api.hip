@@ -372,8 +372,9 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   timing_mark(c, "gs_fp64");
   HIPC(launch_finish(c, C, p));
   // cache this step's impulses (GeometricImpactFrictionMap.cpp:419); an empty active set
-  // leaves the cache empty
-  d.prev_count = 0;
+  // empties the cache (:286-288); otherwise a step without caching (NONE) leaves it
+  // untouched, as the reference's map never touches its cache in that mode
+  if (p->warm_start != SCX_WARM_NONE || C == 0) d.prev_count = 0;
   if (p->warm_start != SCX_WARM_NONE && C > 0) {
     if (C > d.cap_cache) {
       const uint32_t want = C + C / 4 + 1024;

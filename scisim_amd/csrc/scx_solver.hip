@@ -184,6 +184,7 @@ struct GSArgs {
   int phase;                // 0: only phase, 1: FP32 phase, 2: FP64 polish after FP32
   int apply_init;           // reset velocities to v0 and apply the records' impulses
   uint32_t* lmap;           // per body slot: owning endpoint in the LDS solve (all ones when idle)
+  uint32_t tail_col;        // colours [tail_col, ncolours) are swept by workgroup 0 alone (small)
 };
 
 // One contact of a colour pass. copy_best: first store the contact's impulse as it was
@@ -373,8 +374,20 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
     for (it = 1; it <= max_iters; ++it) {
       const bool copy = copy_pending;
       copy_pending = false;
-      for (uint32_t col = 0; col < ncol; ++col) {
+      const uint32_t tail = min(A.tail_col, ncol);
+      for (uint32_t col = 0; col < tail; ++col) {
         for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false, copy);
+        grid_barrier(A.bar);
+      }
+      if (tail < ncol) {
+        // the last, small colours (typically the contacts of a few bodies of high degree):
+        // one workgroup sweeps them in order with workgroup barriers, then one grid barrier,
+        // instead of a grid barrier per colour; the sweep order is unchanged
+        if (blockIdx.x == 0)
+          for (uint32_t col = tail; col < ncol; ++col) {
+            for (uint32_t k = off[col] + threadIdx.x; k < off[col + 1]; k += kGSBlock) process<D, Real>(A, k, false, copy);
+            __syncthreads();
+          }
         grid_barrier(A.bar);
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
@@ -787,14 +800,26 @@ constexpr size_t kGSLdsMax = 124 * 1024;
 
 template <int D, typename Real>
 static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint32_t* col_off) {
-  // per sweep: one barrier interval per colour and one for the residual over all contacts
-  uint32_t widths[kMaxColours + 1];
-  int nw = 0;
+  // Tail: the suffix of colours no wider than one workgroup, swept by one workgroup (no
+  // grid barrier between them) when the launch has several workgroups and the suffix has
+  // at least two colours.
+  int ncol = 0;
   for (int b = 0; b < kMaxColours; ++b)
+    if (col_off[b + 1] > col_off[b]) ncol = b + 1;
+  int tail = ncol;
+  while (tail > 0 && col_off[tail] - col_off[tail - 1] <= (uint32_t)kGSBlock) --tail;
+  if (ncol - tail < 2) tail = ncol;
+  // per sweep: one barrier interval per colour before the tail, one for the tail and one
+  // for the residual over all contacts
+  uint32_t widths[kMaxColours + 2];
+  int nw = 0;
+  for (int b = 0; b < tail; ++b)
     if (col_off[b + 1] > col_off[b]) widths[nw++] = col_off[b + 1] - col_off[b];
+  if (tail < ncol) widths[nw++] = 1;
   widths[nw++] = A.C;
   const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks);
   const int blocks = std::max(1, std::min(max_blocks, want));
+  A.tail_col = (blocks > 1) ? (uint32_t)tail : (uint32_t)kMaxColours;
   const size_t lds = LdsLayout<D, Real>::bytes(A.C);
   if (blocks == 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap) {
     // the whole solve in one workgroup's LDS

@@ -326,3 +326,33 @@ def test_multi_workgroup_barrier_paths_match_oracle(monkeypatch, dim):
     _say("16 workgroups", dim, worst, "fp32+polish", st)
     assert worst["alpha"] < 1e-8 and worst["beta"] < 1e-8 and worst["v"] < 1e-8 and worst["q"] < 1e-11
     assert st["ncontacts"] > 0 and np.isfinite(st["error"])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_a_step_without_caching_leaves_the_cache_on_the_device(dim):
+    """As the oracle: a cold step (NONE) leaves the device cache as it was, so the next
+    caching step warm-starts a resting body with no sweep; an empty active set empties it."""
+    ctx = scx.Context(dim, 16, max_planes=4)
+    g = [0.0, -9.81, 0.0][:dim]
+    q = np.zeros(dim)
+    q[1] = 0.1
+    ctx.set_planes(np.zeros(dim), np.eye(dim)[1])
+    ctx.set_bodies(q, np.zeros(dim), [0.1], [2.0])
+
+    def step(mode):
+        return ctx.step(scx.step_params(**_params(dim, mu=0.5, tol=1e-12, warm_start=mode)))
+
+    for _ in range(3):
+        step(scx.WARM_NORMAL)
+    assert step(scx.WARM_NONE)["sweeps"] >= 1
+    assert step(scx.WARM_NORMAL)["sweeps"] == 0
+    lifted = q.copy()
+    lifted[1] = 5.0
+    ctx.set_bodies(lifted, np.zeros(dim), [0.1], [2.0])  # same body count: the cache is kept
+    assert step(scx.WARM_NONE)["ncontacts"] == 0         # empty active set: the cache is emptied
+    v = np.zeros(dim)
+    v[1] = -9.81 * 0.01
+    ctx.set_bodies(q, v, [0.1], [2.0])
+    assert step(scx.WARM_NORMAL)["sweeps"] >= 1
+    assert step(scx.WARM_NORMAL)["sweeps"] == 0
+    ctx.close()

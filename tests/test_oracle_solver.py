@@ -260,6 +260,29 @@ def test_cached_impulses_warm_start_a_resting_body(dim, mode):
     assert all(s["sweeps"] == 0 for s in out[mode][0][2:])
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_a_step_without_caching_leaves_the_cache(dim):
+    """The map writes its cache only when it caches: after a cold step (NONE) the next
+    caching step still warm-starts from the last cached impulse (no sweep for the resting
+    body), while an empty active set empties the cache."""
+    g = [0.0, -9.81, 0.0][:dim]
+    q = np.zeros(dim)
+    q[1] = 0.1
+    sim = oracle.Sim(dim, q, np.zeros(dim), [0.1], [2.0], np.zeros(dim), np.eye(dim)[1])
+    for _ in range(3):
+        sim.step(dt=0.01, g=g, mu=0.5, tol=1e-12, warm_start=2)
+    assert sim.step(dt=0.01, g=g, mu=0.5, tol=1e-12, warm_start=0)["sweeps"] >= 1
+    assert sim.step(dt=0.01, g=g, mu=0.5, tol=1e-12, warm_start=2)["sweeps"] == 0
+    sim.q[1] = 5.0  # lift it off the floor: empty active set
+    sim.v[:] = 0.0
+    assert sim.step(dt=0.01, g=g, mu=0.5, tol=1e-12, warm_start=0)["ncontacts"] == 0
+    sim.q[1] = 0.1
+    sim.v[:] = 0.0
+    sim.v[1] = -9.81 * 0.01  # resting: v0 = g dt, as after any step on the floor
+    assert sim.step(dt=0.01, g=g, mu=0.5, tol=1e-12, warm_start=2)["sweeps"] >= 1  # cache was emptied
+    assert sim.step(dt=0.01, g=g, mu=0.5, tol=1e-12, warm_start=2)["sweeps"] == 0
+
+
 def test_normal_and_friction_cache_is_2d_only():
     sim = oracle.Sim(3, [0, 0.1, 0], np.zeros(3), [0.1], [1.0], np.zeros(3), [0, 1.0, 0])
     with pytest.raises(RuntimeError):
