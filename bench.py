@@ -20,7 +20,10 @@ Workloads (synthetic, seeded):
 
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
 narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final
-update), with the state resident in HBM. value = bodies * steps / wall time of the
+update), with the state resident in HBM. Sweep limit (the reference scenes' two
+settings): 50000 for cfg1-cfg3, so that every step reaches the tolerance; 5000 for the
+ensemble, whose steps would otherwise run for minutes (config.steps_at_tolerance says how
+many did). value = bodies * steps / wall time of the
 timed steps. The dominant kernel's roofline uses algorithmic bytes per launch
 (scisim_amd/roofline.py) over its average duration from HIP events on the library's stream. The CPU
 baseline is the FP64 oracle (a restatement of the reference path) timed on a bounded
@@ -42,7 +45,7 @@ def make_scene(cfg, seed):
     from scisim_amd import scenes
     if cfg == "cfg1":
         s = scenes.ball2d_rsa(n=1048576, rmin=0.02, rmax=0.06, phi=0.55, seed=seed)
-        prm = dict(dt=0.005, g=[0.0, -9.81], mu=0.3, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1,
+        prm = dict(dt=0.005, g=[0.0, -9.81], mu=0.3, CoR=0.0, tol=1e-6, max_iters=50000, eval_every=1,
                    integrator=0)
         desc = {"workload": "ball2d_rsa_1M", "bodies": 1048576, "dim": 2, "radii": "U[0.02,0.06]",
                 "packing": "RSA phi=0.55", "planes": 4, "mu": 0.3, "CoR": 0.0, "dt": 0.005,
@@ -59,7 +62,7 @@ def make_scene(cfg, seed):
                 "dt": 0.005, "integrator": "verlet", "tol": 1e-6}
     else:
         s = scenes.spheres_pile(n=262144, rmed=0.05, sigma=0.25, seed=seed)
-        prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1,
+        prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, CoR=0.0, tol=1e-6, max_iters=50000, eval_every=1,
                    integrator=1)
         desc = {"workload": "spheres3d_pile_262k" + ("_learned_warm_start" if cfg == "cfg3" else ""),
                 "bodies": 262144, "dim": 3, "radii": "lognormal(0.05,0.25)",

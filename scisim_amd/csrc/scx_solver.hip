@@ -285,10 +285,14 @@ template <int D, typename Real>
 __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize) {
   __shared__ double s_red[kGSBlock / 64];
   double m = 0.0;
-  if (A.phase == 1)  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
+  // read-only loop: unrolled so that the loads of several contacts are in flight together
+  if (A.phase == 1) {  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
+#pragma unroll 4
     for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float>(A, k));
-  else
+  } else {
+#pragma unroll 4
     for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real>(A, k));
+  }
   m = wave_reduce_max_f64(m);
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
   __syncthreads();
