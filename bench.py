@@ -101,18 +101,21 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log, compare=True):
         return out
     # every mode starts from the same state, after one transported-cache step that primes
     # the cache (a cold step leaves the cache empty, so the cold start runs last)
+    # (the cold start needs tens of thousands of sweeps per step: fewer steps of it)
     for name, mode in (("transport", scx.WARM_TRANSPORT), ("learned", scx.WARM_LEARNED), ("none", scx.WARM_NONE)):
         ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
         ctx.step(tp)
+        nsteps = args.steps if mode != scx.WARM_NONE else min(args.steps, 5)
         sw, ok = [], 0
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for k in range(nsteps):
             st = ctx.step(scx.step_params(precision=precision, **dict(prm, warm_start=mode)))
             sw.append(st["sweeps"])
             ok += int(st["succeeded"])
+            log(name, "step", k, "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
         dt = time.perf_counter() - t0
-        out[name] = {"mean_sweeps": float(np.mean(sw)), "converged_steps": ok,
-                     "body_steps_per_s": s["r"].size * args.steps / dt}
+        out[name] = {"steps": nsteps, "mean_sweeps": float(np.mean(sw)), "converged_steps": ok,
+                     "body_steps_per_s": s["r"].size * nsteps / dt}
         log(name, out[name], sw)
     ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
     ctx.step(tp)
@@ -149,22 +152,25 @@ def cpu_baseline(s, state, prm, budget_s):
             k += 1
         return {"value": done / t_all, "unit": "body-steps/s", "cores": 1, "kind": "port",
                 "sample": "FP64 oracle, natural-order GS, one step of each of the first %d scenes alone from the "
-                          "state at the start of the timed steps (%d bodies, %d of them reached the tolerance), "
+                          "state in the middle of the timed steps (%d bodies, %d of them reached the tolerance), "
                           "%.1f s" % (k, done, ok, t_all)}
     n = s["r"].size
     P = s["px"].size // D
-    t2, st2 = one(0, n, 0, P, prm["mu"], 2)  # set-up plus two sweeps
-    if st2["succeeded"] or t2 >= budget_s:
-        t, st, m = t2, st2, 2
-    else:
-        t1, _ = one(0, n, 0, P, prm["mu"], 1)
-        per = max(t2 - t1, 1e-6)
-        m = int(min(prm["max_iters"], max(2, (budget_s - t2 - t1) / per)))
-        t, st = one(0, n, 0, P, prm["mu"], m)
+    # set-up plus one sweep, then sixteen: the time per sweep sets the limit of the step
+    # that uses the rest of the budget
+    t1, st1 = one(0, n, 0, P, prm["mu"], 1)
+    t, st, m = t1, st1, 1
+    if not st1["succeeded"] and t1 < budget_s:
+        t16, st16 = one(0, n, 0, P, prm["mu"], 16)
+        t, st, m = t16, st16, 16
+        if not st16["succeeded"] and t1 + t16 < budget_s:
+            per = max((t16 - t1) / 15.0, 1e-7)
+            m = int(min(prm["max_iters"], max(16, (budget_s - t1 - t16 - t1) / per)))
+            t, st = one(0, n, 0, P, prm["mu"], m)
     how = ("reached the tolerance in %d sweeps" % st["sweeps"]) if st["succeeded"] else (
         "cut at %d sweeps before the tolerance (residual %.1e), so the rate is an upper bound" % (m, st["error"]))
     return {"value": n / t, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle, natural-order GS from a cold start, one step from the state at the start of the "
+            "sample": "FP64 oracle, natural-order GS from a cold start, one step from the state in the middle of the "
                       "timed steps, %s; %.1f s" % (how, t)}
 
 
@@ -302,6 +308,14 @@ def main():
         kernel, launches, kbytes, kms = "k_pairs (grid neighbour sweep + FP64 narrow phase)", 2, pairs_bytes, pairs_ms
     else:
         kernel, launches, kbytes, kms = "k_gs (coloured Gauss-Seidel, persistent)", 1 + int(fp32), gs_bytes, gs_ms
+    # measured HBM bytes per launch of this kernel on this workload (rocprofv3 byte counters
+    # of the same command), when they exist
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "scisim_amd", "measured_traffic.json")) as f:
+            traffic = json.load(f).get(desc["workload"], {}).get(kernel.split()[0], {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        traffic = None
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     if "ensemble" in s:
         cnt, err = ctx.scene_stats()
@@ -327,7 +341,7 @@ def main():
                        phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()},
                        **({"warm_start_comparison": comparison} if comparison else {})),
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": rl.HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / rl.HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / rl.HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": kbytes / (launches * args.steps),
                      "us_per_launch": 1000.0 * kms / (launches * args.steps)},
     }
