@@ -187,13 +187,32 @@ struct GSArgs {
   uint32_t tail_col;        // colours [tail_col, ncolours) are swept by workgroup 0 alone (small)
 };
 
-// One contact of a colour pass. copy_best: first store the contact's impulse as it was
-// before this sweep into rbest (the previous sweep's iterate was the best one so far).
+// A contact's record and impulse, loaded ahead of its colour pass. Only the thread that
+// updates a contact ever writes its impulse, and the record is read-only during the
+// solve, so the load may be issued before the grid barrier that opens the pass.
 template <int D, typename Real>
-__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init, bool copy_best = false) {
-  using V = typename Vec4<Real>::type;
+struct Loaded {
   Rec<D, Real> R;
-  load_rec<D, Real>(A.c, k, R);
+  Real r[3];
+};
+
+template <int D, typename Real>
+__device__ __forceinline__ void load_contact(const GSArgs<Real>& A, uint32_t k, Loaded<D, Real>& L) {
+  load_rec<D, Real>(A.c, k, L.R);
+  const Real* rp = A.c.r + 3 * (size_t)k;
+  L.r[0] = rp[0];
+  L.r[1] = rp[1];
+  L.r[2] = rp[2];
+}
+
+// One contact of a colour pass from its loaded record. copy_best: first store the
+// contact's impulse as it was before this sweep into rbest (the previous sweep's iterate
+// was the best one so far).
+template <int D, typename Real>
+__device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const Loaded<D, Real>& L, bool init,
+                                       bool copy_best) {
+  using V = typename Vec4<Real>::type;
+  const Rec<D, Real>& R = L.R;
   const bool hasb = R.b != kNoBody;
   V* bv = reinterpret_cast<V*>(A.body);
   V* bw = bv + 1;  // 3D: the angular half of the same record
@@ -205,7 +224,7 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
     if (D == 3) wb = bw[ib];
   }
   Real* rp = A.c.r + 3 * (size_t)k;
-  const Real r[3] = {rp[0], rp[1], rp[2]};
+  const Real r[3] = {L.r[0], L.r[1], L.r[2]};
   if (copy_best) {
     Real* rb = A.rbest + 3 * (size_t)k;
     rb[0] = r[0];
@@ -258,6 +277,14 @@ __device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool 
       bw[ib] = wb;
     }
   }
+}
+
+// One contact of a colour pass (record loaded here).
+template <int D, typename Real>
+__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init, bool copy_best = false) {
+  Loaded<D, Real> L;
+  load_contact<D, Real>(A, k, L);
+  update<D, Real>(A, k, L, init, copy_best);
 }
 
 template <int D, typename Real, typename T = double>
@@ -380,8 +407,17 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       const bool copy = copy_pending;
       copy_pending = false;
       const uint32_t tail = min(A.tail_col, ncol);
+      // the first contact of each colour pass is loaded before the barrier that opens it,
+      // so that only the body gathers wait for the barrier
+      Loaded<D, Real> next;
+      if (tail > 0 && off[0] + gtid < off[1]) load_contact<D, Real>(A, off[0] + gtid, next);
       for (uint32_t col = 0; col < tail; ++col) {
-        for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false, copy);
+        uint32_t k = off[col] + gtid;
+        if (k < off[col + 1]) {
+          update<D, Real>(A, k, next, false, copy);
+          for (k += gsize; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false, copy);
+        }
+        if (col + 1 < tail && off[col + 1] + gtid < off[col + 2]) load_contact<D, Real>(A, off[col + 1] + gtid, next);
         grid_barrier(A.bar);
       }
       if (tail < ncol) {
