@@ -324,11 +324,19 @@ def main():
         traffic = None
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     if "ensemble" in s:
-        cnt, err = ctx.scene_stats()
+        cnt, err, ssw = ctx.scene_stats()
         desc["step_latency_ms_p50"] = float(np.percentile(step_ms, 50))
         desc["step_latency_ms_p99"] = float(np.percentile(step_ms, 99))
         desc["scene_contacts_p50_p99"] = [float(np.percentile(cnt, 50)), float(np.percentile(cnt, 99))]
         desc["scene_residual_max"] = float(err.max())
+        desc["scenes_at_tolerance_last_step"] = int((err < prm["tol"]).sum())
+        # per-scene latency of the last step: its share of the sweeps (a scene is frozen once it
+        # meets the tolerance) plus everything outside the sweeps
+        gs_last = sum(ms for name, ms in tm if name.startswith("gs_"))
+        other = step_ms[-1] - gs_last
+        lat = other + gs_last * ssw / max(1, sweeps[-1])
+        desc["scene_sweeps_p50_p99"] = [float(np.percentile(ssw, 50)), float(np.percentile(ssw, 99))]
+        desc["scene_latency_ms_p50_p99_last_step"] = [float(np.percentile(lat, 50)), float(np.percentile(lat, 99))]
     out = {
         "metric": "body-steps/sec at FB residual<=1e-6",
         "value": value,

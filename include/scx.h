@@ -136,14 +136,16 @@ int scx_set_planes(scx_ctx* ctx, uint32_t nplanes, const double* x, const double
  * to scx_set_planes (at most 64), and has friction coefficient mu[k] (scx_step_params.mu
  * is then unused). Bodies of different scenes never interact; within a scene the active
  * set, its order and the contact data are those of the scene alone. The Gauss-Seidel
- * solve runs until every scene meets the tolerance (one residual, the max over all
- * contacts). Call after scx_set_bodies and scx_set_planes; a new body or plane count
+ * solve runs until every scene meets the tolerance; a scene that meets it is frozen (its
+ * contacts are no longer updated) for the rest of the solve. Call after scx_set_bodies and scx_set_planes; a new body or plane count
  * clears the ensemble; nscenes = 0 returns to one scene. */
 int scx_set_scenes(scx_ctx* ctx, uint32_t nscenes, const uint32_t* body_off, const uint32_t* plane_off,
                    const double* mu);
-/* Per-scene results of the last step of an ensemble: contacts and residual (max over the
- * scene's contacts of ||FB||^2, in FP64). Either array may be NULL. */
-int scx_get_scene_stats(scx_ctx* ctx, uint32_t nscenes, uint32_t* ncontacts, double* error);
+/* Per-scene results of the last step of an ensemble: contacts, residual (max over the
+ * scene's contacts of ||FB||^2, in FP64) and the sweep at which the scene met the tolerance
+ * (its contacts are not updated after that; a scene that never met it reports the step's
+ * total sweeps). Any array may be NULL. */
+int scx_get_scene_stats(scx_ctx* ctx, uint32_t nscenes, uint32_t* ncontacts, double* error, uint32_t* sweeps);
 int scx_get_state(scx_ctx* ctx, double* q, double* v, double* omega);
 
 /* One GeometricImpactFrictionMap step on the resident state. stats may be NULL. */

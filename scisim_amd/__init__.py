@@ -98,7 +98,7 @@ def lib():
         L.scx_set_bodies.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, _pd, _pd, _pd, _pd]
         L.scx_set_planes.argtypes = [_vp, ctypes.c_uint32, _pd, _pd]
         L.scx_set_scenes.argtypes = [_vp, ctypes.c_uint32, _pu32, _pu32, _pd]
-        L.scx_get_scene_stats.argtypes = [_vp, ctypes.c_uint32, _pu32, _pd]
+        L.scx_get_scene_stats.argtypes = [_vp, ctypes.c_uint32, _pu32, _pd, _pu32]
         L.scx_get_state.argtypes = [_vp, _pd, _pd, _pd]
         L.scx_step.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.POINTER(ScxStepStats)]
         L.scx_run.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.c_uint32, ctypes.POINTER(ScxStepStats)]
@@ -214,12 +214,15 @@ class Context:
         self._check(lib().scx_set_scenes(self._h, ns, _p(bo, _pu32), _p(po, _pu32), _p(mu)), "scx_set_scenes")
 
     def scene_stats(self):
-        """(contacts, residual) per scene of the last step of an ensemble."""
+        """(contacts, residual, sweeps to the tolerance) per scene of the last step of an
+        ensemble."""
         ns = self.nscenes
         cnt = np.zeros(ns, dtype=np.uint32)
         err = np.zeros(ns)
-        self._check(lib().scx_get_scene_stats(self._h, ns, _p(cnt, _pu32), _p(err)), "scx_get_scene_stats")
-        return cnt, err
+        sw = np.zeros(ns, dtype=np.uint32)
+        self._check(lib().scx_get_scene_stats(self._h, ns, _p(cnt, _pu32), _p(err), _p(sw, _pu32)),
+                    "scx_get_scene_stats")
+        return cnt, err, sw
 
     def get_state(self):
         q = np.empty(self.dim * self.n)

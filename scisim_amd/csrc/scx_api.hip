@@ -166,6 +166,7 @@ int alloc_contacts(scx_ctx* c, uint32_t C) {
   HIPC(dalloc(&d.cwt, C));
   HIPC(dalloc(&d.cforcing, C));
   HIPC(dalloc(&d.cmu, C));
+  HIPC(dalloc(&d.cscene, C));
   HIPC(dalloc(&d.calpha, C));
   HIPC(dalloc(&d.cbeta, (size_t)(D - 1) * C));
   HIPC(dalloc(&d.crinit, (size_t)D * C));
@@ -209,6 +210,9 @@ void free_all(scx_ctx* c) {
   dfree(&d.bscene);
   dfree(&d.scene_poff);
   dfree(&d.scene_cnt);
+  dfree(&d.cscene);
+  dfree(&d.scene_acc);
+  dfree(&d.scene_sweeps);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -702,6 +706,8 @@ int scx_set_scenes(scx_ctx* c, uint32_t nscenes, const uint32_t* body_off, const
     HIPC(dalloc(&d.scene_shift, (size_t)3 * nscenes));
     HIPC(dalloc(&d.scene_err, nscenes));
     HIPC(dalloc(&d.scene_cnt, nscenes));
+    HIPC(dalloc(&d.scene_acc, (size_t)2 * nscenes));
+    HIPC(dalloc(&d.scene_sweeps, nscenes));
     c->cap_scenes = nscenes;
   }
   HIPC(dalloc(&d.bscene, c->cap_bodies));
@@ -716,7 +722,7 @@ int scx_set_scenes(scx_ctx* c, uint32_t nscenes, const uint32_t* body_off, const
   return SCX_OK;
 }
 
-int scx_get_scene_stats(scx_ctx* c, uint32_t nscenes, uint32_t* ncontacts, double* error) {
+int scx_get_scene_stats(scx_ctx* c, uint32_t nscenes, uint32_t* ncontacts, double* error, uint32_t* sweeps) {
   if (!c) return SCX_E_ARG;
   if (c->nscenes == 0 || nscenes != c->nscenes)
     return fail(c, SCX_E_ARG, "scx_get_scene_stats: the context holds %u scenes", c->nscenes);
@@ -726,7 +732,18 @@ int scx_get_scene_stats(scx_ctx* c, uint32_t nscenes, uint32_t* ncontacts, doubl
   if (ncontacts)
     HIPC(hipMemcpyAsync(ncontacts, d.scene_cnt, sizeof(uint32_t) * nscenes, hipMemcpyDeviceToHost, c->stream));
   if (error) HIPC(hipMemcpyAsync(error, d.scene_err, sizeof(double) * nscenes, hipMemcpyDeviceToHost, c->stream));
+  if (sweeps) {
+    if (c->last_ncontacts > 0)
+      HIPC(hipMemcpyAsync(sweeps, d.scene_sweeps, sizeof(uint32_t) * nscenes, hipMemcpyDeviceToHost, c->stream));
+    else
+      for (uint32_t k = 0; k < nscenes; ++k) sweeps[k] = 0;
+  }
+  StepCounters ctr{};
+  HIPC(hipMemcpyAsync(&ctr, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
+  if (sweeps && c->last_ncontacts > 0)  // a scene that never met the tolerance ran every sweep
+    for (uint32_t k = 0; k < nscenes; ++k)
+      if (sweeps[k] == kNoBody) sweeps[k] = ctr.sweeps;
   return SCX_OK;
 }
 

@@ -114,6 +114,9 @@ struct DeviceBuffers {
   double* scene_err = nullptr;     // S: per-scene residual (scx_get_scene_stats)
   uint32_t* scene_cnt = nullptr;   // S: per-scene contacts
   double* cmu = nullptr;           // per contact friction coefficient (cap_contacts)
+  uint32_t* cscene = nullptr;      // per contact in colour order: its scene (cap_contacts)
+  unsigned long long* scene_acc = nullptr;  // 2 S: per-scene residual bits of the sweeps
+  uint32_t* scene_sweeps = nullptr;         // S: sweep at which each scene met the tolerance
   // broad phase
   double* aabb_min = nullptr;  // dim*N
   double* aabb_max = nullptr;

@@ -185,7 +185,32 @@ struct GSArgs {
   int apply_init;           // reset velocities to v0 and apply the records' impulses
   uint32_t* lmap;           // per body slot: owning endpoint in the LDS solve (all ones when idle)
   uint32_t tail_col;        // colours [tail_col, ncolours) are swept by workgroup 0 alone (small)
+  // ensembles (else cscene == nullptr): per contact (colour order) its scene; per scene the
+  // residual of the last two evaluations (max ||FB||^2 bits, double-buffered) and the sweep
+  // at which it met the tolerance (kNoBody until then). A scene that meets the tolerance is
+  // frozen: its contacts are skipped until the solve ends.
+  const uint32_t* cscene;
+  unsigned long long* scene_acc;  // 2 x nscenes
+  uint32_t* scene_sweeps;         // nscenes
+  uint32_t nscenes;
 };
+
+// True if contact k belongs to a scene whose last evaluated residual is below stop.
+template <typename Real>
+__device__ __forceinline__ bool frozen(const GSArgs<Real>& A, uint32_t k, uint32_t eb, double stop) {
+  if (!A.cscene) return false;
+  const unsigned long long bits =
+      __hip_atomic_load(A.scene_acc + (size_t)eb * A.nscenes + A.cscene[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double((long long)bits) < stop;
+}
+
+// A frozen contact keeps its impulse; the lazy best-iterate copy still stores it.
+template <typename Real>
+__device__ __forceinline__ void keep(const GSArgs<Real>& A, uint32_t k, bool copy_best) {
+  if (!copy_best) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
+}
 
 // A contact's record and impulse, loaded ahead of its colour pass. Only the thread that
 // updates a contact ever writes its impulse, and the record is read-only during the
@@ -307,13 +332,49 @@ __device__ __forceinline__ T contact_err(const GSArgs<Real>& A, uint32_t k) {
   return fb_err<D, T>((T)R.mu, r, u);
 }
 
-// Grid-wide max of the residual; identical in every thread after the barrier.
+// Per-scene max of one contact's residual (ensembles): lanes of a wave that share a scene
+// (the usual case: contacts are in spatial order within a colour) combine before one atomic.
+__device__ __forceinline__ void scene_max(unsigned long long* acc, uint32_t scene, double e, bool valid) {
+  const unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(e) : 0ull;
+  const uint32_t first = __shfl(valid ? scene : 0xFFFFFFFFu, __ffsll((long long)__ballot(valid)) - 1, 64);
+  const bool same = __all(!valid || scene == first);
+  if (same) {
+    unsigned long long m = bits;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(m, off, 64);
+      m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && __ballot(valid) != 0) atomicMax(acc + first, m);
+  } else if (valid) {
+    atomicMax(acc + scene, bits);
+  }
+}
+
+// Grid-wide max of the residual; identical in every thread after the barrier. Ensembles:
+// also the per-scene max into buffer ev & 1 (the other buffer is cleared for the next
+// evaluation), and the sweep at which each scene meets `stop` (it sweeps done in this
+// launch, base those of earlier phases).
 template <int D, typename Real>
-__device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize) {
+__device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize, uint32_t ev = 0,
+                                double stop = 0.0, uint32_t base = 0, uint32_t it = 0) {
   __shared__ double s_red[kGSBlock / 64];
   double m = 0.0;
-  // read-only loop: unrolled so that the loads of several contacts are in flight together
-  if (A.phase == 1) {  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
+  if (A.cscene) {
+    unsigned long long* acc = A.scene_acc + (size_t)(ev & 1) * A.nscenes;
+    unsigned long long* other = A.scene_acc + (size_t)((ev + 1) & 1) * A.nscenes;
+    for (uint32_t sc = gtid; sc < A.nscenes; sc += gsize)
+      __hip_atomic_store(other + sc, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t nk = (A.C + gsize - 1) / gsize * gsize;  // whole waves take part in scene_max
+    for (uint32_t k = gtid; k < nk; k += gsize) {
+      const bool valid = k < A.C;
+      double e = 0.0;
+      if (valid) e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
+      m = fmax(m, e);
+      scene_max(acc, valid ? A.cscene[k] : 0u, e, valid);
+    }
+  } else if (A.phase == 1) {  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
+    // read-only loop: unrolled so that the loads of several contacts are in flight together
 #pragma unroll 4
     for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float>(A, k));
   } else {
@@ -329,6 +390,18 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
     A.partial[blockIdx.x] = b;
   }
   grid_barrier(A.bar);
+  if (A.cscene) {
+    unsigned long long* acc = A.scene_acc + (size_t)(ev & 1) * A.nscenes;
+    for (uint32_t sc = gtid; sc < A.nscenes; sc += gsize) {
+      const double es = __longlong_as_double(
+          (long long)__hip_atomic_load(acc + sc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (es < stop) {
+        if (A.scene_sweeps[sc] == kNoBody) A.scene_sweeps[sc] = base + it;
+      } else if (A.phase == 2) {
+        A.scene_sweeps[sc] = kNoBody;  // an FP32 estimate below the tolerance was not confirmed
+      }
+    }
+  }
   double e = 0.0;
   for (uint32_t b = threadIdx.x; b < gridDim.x; b += kGSBlock) e = fmax(e, A.partial[b]);
   e = wave_reduce_max_f64(e);
@@ -385,8 +458,17 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   const bool track_best = A.phase != 1;
 
   if (A.apply_init) apply_impulses<D, Real>(A, off, ncol, gtid, gsize, false);
-
-  double err = eval_residual<D, Real>(A, gtid, gsize);
+  // the FP32 phase stops on an FP32 estimate with a factor-2 margin; the FP64 phase
+  // that follows decides on the tolerance
+  const double stop = (A.phase == 1) ? 0.5 * A.tol : A.tol;
+  const uint32_t base = (A.phase == 2) ? A.ctr->sweeps_fp32 : 0u;
+  if (A.cscene) {
+    for (uint32_t sc = gtid; sc < 2 * A.nscenes; sc += gsize)
+      __hip_atomic_store(A.scene_acc + sc, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    grid_barrier(A.bar);
+  }
+  uint32_t ev = 0;  // evaluations so far; the last one's scene residuals are in buffer (ev - 1) & 1
+  double err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, 0);
   double best = err;
   // Best iterate (So-bogus returns it when the sweep limit is reached): after an
   // evaluation that lowers the best residual, the next sweep stores every contact's
@@ -399,13 +481,11 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   double ref = best;
   uint32_t ref_it = 0;
   uint32_t it = 0;
-  // the FP32 phase stops on an FP32 estimate with a factor-2 margin; the FP64 phase
-  // that follows decides on the tolerance
-  const double stop = (A.phase == 1) ? 0.5 * A.tol : A.tol;
   if (!(err < stop)) {
     for (it = 1; it <= max_iters; ++it) {
       const bool copy = copy_pending;
       copy_pending = false;
+      const uint32_t eb = (ev - 1) & 1;
       const uint32_t tail = min(A.tail_col, ncol);
       // the first contact of each colour pass is loaded before the barrier that opens it,
       // so that only the body gathers wait for the barrier
@@ -414,8 +494,12 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       for (uint32_t col = 0; col < tail; ++col) {
         uint32_t k = off[col] + gtid;
         if (k < off[col + 1]) {
-          update<D, Real>(A, k, next, false, copy);
-          for (k += gsize; k < off[col + 1]; k += gsize) process<D, Real>(A, k, false, copy);
+          if (frozen(A, k, eb, stop)) keep(A, k, copy);
+          else update<D, Real>(A, k, next, false, copy);
+          for (k += gsize; k < off[col + 1]; k += gsize) {
+            if (frozen(A, k, eb, stop)) keep(A, k, copy);
+            else process<D, Real>(A, k, false, copy);
+          }
         }
         if (col + 1 < tail && off[col + 1] + gtid < off[col + 2]) load_contact<D, Real>(A, off[col + 1] + gtid, next);
         grid_barrier(A.bar);
@@ -426,13 +510,16 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         // instead of a grid barrier per colour; the sweep order is unchanged
         if (blockIdx.x == 0)
           for (uint32_t col = tail; col < ncol; ++col) {
-            for (uint32_t k = off[col] + threadIdx.x; k < off[col + 1]; k += kGSBlock) process<D, Real>(A, k, false, copy);
+            for (uint32_t k = off[col] + threadIdx.x; k < off[col + 1]; k += kGSBlock) {
+              if (frozen(A, k, eb, stop)) keep(A, k, copy);
+              else process<D, Real>(A, k, false, copy);
+            }
             __syncthreads();
           }
         grid_barrier(A.bar);
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
-        err = eval_residual<D, Real>(A, gtid, gsize);
+        err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, it);
         if (err < best) {
           best = err;
           copy_pending = track_best;
@@ -862,7 +949,7 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
   const int blocks = std::max(1, std::min(max_blocks, want));
   A.tail_col = (blocks > 1) ? (uint32_t)tail : (uint32_t)kMaxColours;
   const size_t lds = LdsLayout<D, Real>::bytes(A.C);
-  if (blocks == 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap) {
+  if (blocks == 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap && !A.cscene) {
     // the whole solve in one workgroup's LDS
     hipError_t e = hipFuncSetAttribute((const void*)k_gs_lds<D, Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -883,6 +970,17 @@ int gs_max_blocks(scx_ctx* c) {
   return per_cu * c->num_cus;
 }
 
+// Ensembles: the scene of each contact in colour order (the scene of its body a).
+__global__ void __launch_bounds__(kBlock) k_contact_scene(uint32_t C, const uint32_t* __restrict__ a_slot,
+                                                          const uint32_t* __restrict__ sid,
+                                                          const uint32_t* __restrict__ bscene,
+                                                          uint32_t* __restrict__ cscene) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t sl = a_slot[k];
+  cscene[k] = bscene[sid ? sid[sl] : sl];
+}
+
 template <int D>
 static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, const scx_step_params* p) {
   hipStream_t s = c->stream;
@@ -895,11 +993,24 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
   hipError_t e = hipGetLastError();
   timing_mark(c, "gs_init");
   if (e != hipSuccess || C == 0) return e;
+  const bool ens = c->nscenes > 0;
+  if (ens) {
+    k_contact_scene<<<(C + kBlock - 1) / kBlock, kBlock, 0, s>>>(C, c->d.c64.a, c->d.sid, c->d.bscene, c->d.cscene);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemsetAsync(c->d.scene_sweeps, 0xFF, sizeof(uint32_t) * c->nscenes, s);
+    if (e != hipSuccess) return e;
+  }
   if (fp32) {
     GSArgs<float> A{c->d.c32, c->d.bv32, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
                     reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
     A.lmap = c->d.lmap;
+    if (ens) {
+      A.cscene = c->d.cscene;
+      A.scene_acc = c->d.scene_acc;
+      A.scene_sweeps = c->d.scene_sweeps;
+      A.nscenes = c->nscenes;
+    }
     e = coop_gs<D, float>(c, A, c->gs_blocks, col_off);
     if (e != hipSuccess) return e;
     timing_mark(c, "gs_fp32");
@@ -913,6 +1024,12 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
   B.lmap = c->d.lmap;
+  if (ens) {
+    B.cscene = c->d.cscene;
+    B.scene_acc = c->d.scene_acc;
+    B.scene_sweeps = c->d.scene_sweeps;
+    B.nscenes = c->nscenes;
+  }
   return coop_gs<D, double>(c, B, c->gs_blocks, col_off);
 }
 

@@ -83,9 +83,11 @@ def test_ensemble_2d_matches_each_scene_alone():
     assert st["succeeded"]
     con = ens.contacts()
     q_e, v_e, _ = ens.get_state()
-    cnt, err = ens.scene_stats()
-    _say("scene contacts", cnt, "residuals", err)
+    cnt, err, sw = ens.scene_stats()
+    _say("scene contacts", cnt, "residuals", err, "sweeps", sw)
     assert np.all(err < 1e-14)
+    # each scene stops at its own sweep (it is frozen once it meets the tolerance)
+    assert np.all(sw <= st["sweeps"]) and sw.max() == st["sweeps"]
     solo = scx.Context(2, int(max(p["r"].size for p in parts)), max_planes=8)
     for k, p in enumerate(parts):
         b0, b1 = int(bo[k]), int(bo[k + 1])
@@ -163,8 +165,8 @@ def test_ensemble_arguments_are_checked():
         c.set_scenes([0, n], [0, 4], [-0.5])  # negative mu
     c.set_scenes([0, 10, n], [0, 0, 4], [0.5, 0.1])
     st = c.step(scx.step_params(**_params(2, tol=1e-10)))
-    cnt, err = c.scene_stats()
-    assert cnt.sum() == st["ncontacts"] and np.all(err < 1e-10)
+    cnt, err, sw = c.scene_stats()
+    assert cnt.sum() == st["ncontacts"] and np.all(err < 1e-10) and np.all(sw <= st["sweeps"])
     c.set_scenes([], [], [])  # back to one scene
     with pytest.raises(scx.ScxError):
         c.scene_stats()
