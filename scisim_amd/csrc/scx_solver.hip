@@ -204,6 +204,42 @@ __device__ __forceinline__ bool frozen(const GSArgs<Real>& A, uint32_t k, uint32
   return __longlong_as_double((long long)bits) < stop;
 }
 
+// Frozen scenes as a bit mask in LDS, rebuilt by every workgroup after each evaluation,
+// so that the sweeps test a contact with one LDS read (ensembles of up to kSceneMaskBits
+// scenes; larger ones read the residuals from global memory).
+constexpr uint32_t kSceneMaskBits = 16384;
+
+template <typename Real>
+__device__ __forceinline__ void build_frozen_mask(const GSArgs<Real>& A, uint32_t eb, double stop, uint32_t* mask) {
+  // one scene per thread, one 64-bit ballot per wave (blockDim is a multiple of 64)
+  const uint32_t words = (A.nscenes + 31) / 32;
+  for (uint32_t b0 = 0; b0 < A.nscenes; b0 += blockDim.x) {
+    const uint32_t sc = b0 + threadIdx.x;
+    bool f = false;
+    if (sc < A.nscenes) {
+      const unsigned long long v = __hip_atomic_load(A.scene_acc + (size_t)eb * A.nscenes + sc, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+      f = __longlong_as_double((long long)v) < stop;
+    }
+    const unsigned long long bal = __ballot(f);
+    if ((threadIdx.x & 63) == 0) {
+      const uint32_t w = (b0 + threadIdx.x) >> 5;
+      if (w < words) mask[w] = (uint32_t)bal;
+      if (w + 1 < words) mask[w + 1] = (uint32_t)(bal >> 32);
+    }
+  }
+  __syncthreads();
+}
+
+template <typename Real>
+__device__ __forceinline__ bool frozen_fast(const GSArgs<Real>& A, uint32_t k, uint32_t eb, double stop,
+                                            const uint32_t* mask) {
+  if (!A.cscene) return false;
+  if (A.nscenes > kSceneMaskBits) return frozen(A, k, eb, stop);
+  const uint32_t sc = A.cscene[k];
+  return (mask[sc >> 5] >> (sc & 31)) & 1u;
+}
+
 // A frozen contact keeps its impulse; the lazy best-iterate copy still stores it.
 template <typename Real>
 __device__ __forceinline__ void keep(const GSArgs<Real>& A, uint32_t k, bool copy_best) {
@@ -332,6 +368,9 @@ __device__ __forceinline__ T contact_err(const GSArgs<Real>& A, uint32_t k) {
   return fb_err<D, T>((T)R.mu, r, u);
 }
 
+// Scenes whose per-scene residual max is combined in LDS during an evaluation.
+constexpr uint32_t kSceneLdsMax = 2048;
+
 // Per-scene max of one contact's residual (ensembles): lanes of a wave that share a scene
 // (the usual case: contacts are in spatial order within a colour) combine before one atomic.
 __device__ __forceinline__ void scene_max(unsigned long long* acc, uint32_t scene, double e, bool valid) {
@@ -365,13 +404,33 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
     unsigned long long* other = A.scene_acc + (size_t)((ev + 1) & 1) * A.nscenes;
     for (uint32_t sc = gtid; sc < A.nscenes; sc += gsize)
       __hip_atomic_store(other + sc, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t nk = (A.C + gsize - 1) / gsize * gsize;  // whole waves take part in scene_max
-    for (uint32_t k = gtid; k < nk; k += gsize) {
-      const bool valid = k < A.C;
-      double e = 0.0;
-      if (valid) e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
-      m = fmax(m, e);
-      scene_max(acc, valid ? A.cscene[k] : 0u, e, valid);
+    if (A.nscenes <= kSceneLdsMax) {
+      // each workgroup takes one contiguous range of contacts (in a colour, contacts are in
+      // spatial order, so a range touches few scenes), combines per scene in LDS, then
+      // publishes the scenes it touched
+      __shared__ unsigned long long s_acc[kSceneLdsMax];
+      for (uint32_t sc = threadIdx.x; sc < A.nscenes; sc += kGSBlock) s_acc[sc] = 0ull;
+      __syncthreads();
+      const uint32_t per = (A.C + gridDim.x - 1) / gridDim.x;
+      const uint32_t lo = min(A.C, blockIdx.x * per), hi = min(A.C, lo + per);
+#pragma unroll 4
+      for (uint32_t k = lo + threadIdx.x; k < hi; k += kGSBlock) {
+        const double e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
+        m = fmax(m, e);
+        if (e > 0.0) atomicMax(s_acc + A.cscene[k], (unsigned long long)__double_as_longlong(e));
+      }
+      __syncthreads();
+      for (uint32_t sc = threadIdx.x; sc < A.nscenes; sc += kGSBlock)
+        if (s_acc[sc] != 0ull) atomicMax(acc + sc, s_acc[sc]);
+    } else {
+      const uint32_t nk = (A.C + gsize - 1) / gsize * gsize;  // whole waves take part in scene_max
+      for (uint32_t k = gtid; k < nk; k += gsize) {
+        const bool valid = k < A.C;
+        double e = 0.0;
+        if (valid) e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
+        m = fmax(m, e);
+        scene_max(acc, valid ? A.cscene[k] : 0u, e, valid);
+      }
     }
   } else if (A.phase == 1) {  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
     // read-only loop: unrolled so that the loads of several contacts are in flight together
@@ -448,6 +507,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   const uint32_t gsize = gridDim.x * kGSBlock;
   const uint32_t gtid = blockIdx.x * kGSBlock + threadIdx.x;
   __shared__ uint32_t off[kMaxColours + 1];
+  __shared__ uint32_t s_frozen[kSceneMaskBits / 32];
   const uint32_t ncol = A.ctr->ncolours;
   for (uint32_t c = threadIdx.x; c <= ncol; c += kGSBlock) off[c] = A.coloff[c];
   __syncthreads();
@@ -469,6 +529,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   }
   uint32_t ev = 0;  // evaluations so far; the last one's scene residuals are in buffer (ev - 1) & 1
   double err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, 0);
+  if (A.cscene && A.nscenes <= kSceneMaskBits) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
   double best = err;
   // Best iterate (So-bogus returns it when the sweep limit is reached): after an
   // evaluation that lowers the best residual, the next sweep stores every contact's
@@ -494,10 +555,10 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       for (uint32_t col = 0; col < tail; ++col) {
         uint32_t k = off[col] + gtid;
         if (k < off[col + 1]) {
-          if (frozen(A, k, eb, stop)) keep(A, k, copy);
+          if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
           else update<D, Real>(A, k, next, false, copy);
           for (k += gsize; k < off[col + 1]; k += gsize) {
-            if (frozen(A, k, eb, stop)) keep(A, k, copy);
+            if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
             else process<D, Real>(A, k, false, copy);
           }
         }
@@ -511,7 +572,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         if (blockIdx.x == 0)
           for (uint32_t col = tail; col < ncol; ++col) {
             for (uint32_t k = off[col] + threadIdx.x; k < off[col + 1]; k += kGSBlock) {
-              if (frozen(A, k, eb, stop)) keep(A, k, copy);
+              if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
               else process<D, Real>(A, k, false, copy);
             }
             __syncthreads();
@@ -520,6 +581,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
         err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, it);
+        if (A.cscene && A.nscenes <= kSceneMaskBits) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
         if (err < best) {
           best = err;
           copy_pending = track_best;
