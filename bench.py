@@ -199,6 +199,10 @@ def main():
     ap.add_argument("--eval-every", type=int, default=None, help="override the residual period (profiling)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
+                    help="online autotuning of the persistent sweep launch width (scx_set_autotune); auto: on "
+                         "for the piles and the ensemble, whose untimed settling steps train it, off for the "
+                         "circle packing, whose active set grows through new launch classes every few steps")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 plain device copies of BYTES (the reference kernel that scales the HBM "
@@ -236,6 +240,10 @@ def main():
     ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s.get("inertia"))
     if "ensemble" in s:
         ctx.set_scenes(*s["ensemble"])
+    if args.autotune == "auto":
+        args.autotune = "off" if args.config == "cfg1" else "on"
+    ctx.set_autotune(args.autotune == "on")
+    desc["autotune_enabled"] = args.autotune == "on"
     precision = scx.FP32_POLISH if args.precision == "fp32" else scx.FP64
     it = 1
     pile = args.config in ("cfg2", "cfg3", "cfg4")
@@ -270,7 +278,7 @@ def main():
     # timed region: every step synchronises at its end; per-phase device times from HIP events
     ctx.set_timing(True)
     phase_ms = {}
-    pairs_bytes = gs_bytes = 0.0
+    pairs_bytes = gs_bytes = bp_bytes = 0.0
     contacts, sweeps = [], []
     fp32 = precision == scx.FP32_POLISH
     step_ms = []
@@ -283,7 +291,7 @@ def main():
             # is taken out of the timed wall time)
             tc = time.perf_counter()
             sample = ctx.get_state()
-            sample_s = time.perf_counter() - tc
+            sample_s += time.perf_counter() - tc
         ts = time.perf_counter()
         st = ctx.step(params)
         step_ms.append(1000.0 * (time.perf_counter() - ts))
@@ -291,13 +299,29 @@ def main():
         for name, ms in tm:
             phase_ms[name] = phase_ms.get(name, 0.0) + ms
         C = st["ncontacts"]
+        bp_bytes += rl.broadphase_bytes(dim, n, st["npairs"])
         for write_pass in (False, True):
             pairs_bytes += rl.pairs_launch_bytes(dim, n, st["ncells"], st["nentries"], st["npairs"], write_pass)
         s32 = st["sweeps_fp32"] if fp32 else 0
         s64 = st["sweeps"] - s32
-        if fp32:
-            gs_bytes += rl.gs_launch_bytes(dim, C, s32, s32 + 1, True)
-        gs_bytes += rl.gs_launch_bytes(dim, C, s64, s64 + 1, False)
+        if "ensemble" in s:
+            # a scene is frozen once it meets the tolerance: count each scene's contacts only
+            # for the sweeps it took (per-scene stats read outside the timed wall time)
+            tc = time.perf_counter()
+            cnt_k, _, ssw_k = ctx.scene_stats()
+            sample_s += time.perf_counter() - tc
+            cnt_k = cnt_k.astype(np.float64)
+            sw32 = np.minimum(ssw_k, s32).astype(np.float64)
+            sw64 = np.clip(ssw_k.astype(np.int64) - s32, 0, s64).astype(np.float64)
+            if fp32:
+                gs_bytes += float(np.sum(cnt_k * (sw32 * rl.gs_sweep_bytes(dim, True) +
+                                                  (sw32 + 1) * rl.gs_eval_bytes(dim, True))))
+            gs_bytes += float(np.sum(cnt_k * (sw64 * rl.gs_sweep_bytes(dim, False) +
+                                              (sw64 + 1) * rl.gs_eval_bytes(dim, False))))
+        else:
+            if fp32:
+                gs_bytes += rl.gs_launch_bytes(dim, C, s32, s32 + 1, True)
+            gs_bytes += rl.gs_launch_bytes(dim, C, s64, s64 + 1, False)
         contacts.append(C)
         sweeps.append(st["sweeps"])
         converged += int(st["succeeded"])
@@ -323,6 +347,18 @@ def main():
     except (OSError, ValueError):
         traffic = None
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    # the broad phase as a whole: predictor, binning and both neighbour sweeps
+    bp_ms = sum(phase_ms.get(k, 0.0) for k in ("predict", "grid", "pairs_count", "pairs_scan", "pairs_write"))
+    bp_gbs = bp_bytes / (bp_ms * 1e-3) / 1e9 if bp_ms > 0 else 0.0
+    broadphase = {"bound": "hbm", "achieved": bp_gbs, "peak": rl.HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": bp_gbs / rl.HBM_PEAK_GBS, "bytes_per_step": bp_bytes / args.steps,
+                  "us_per_step": 1000.0 * bp_ms / args.steps,
+                  "phases": "predict + grid + pairs_count + pairs_scan + pairs_write",
+                  "bytes": "172 B/body + 8 B/pair (2D), 208 B/body + 8 B/pair (3D)"}
+    tuned = ctx.autotune_table()
+    if tuned:
+        desc["autotune"] = [[e["phase"], e["contacts_half_octave"], e["exponent"], e["blocks"],
+                             round(e["us_per_sweep"], 2), e["settled"]] for e in tuned]
     if "ensemble" in s:
         cnt, err, ssw = ctx.scene_stats()
         desc["step_latency_ms_p50"] = float(np.percentile(step_ms, 50))
@@ -358,6 +394,7 @@ def main():
                      "frac": achieved / rl.HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": kbytes / (launches * args.steps),
                      "us_per_launch": 1000.0 * kms / (launches * args.steps)},
+        "broadphase_roofline": broadphase,
     }
     ctx.close()
     if not args.no_cpu:

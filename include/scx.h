@@ -223,6 +223,23 @@ int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const d
 int scx_set_timing(scx_ctx* ctx, int32_t enable);
 int scx_get_timing(scx_ctx* ctx, uint32_t cap, const char** names, double* ms, uint32_t* count);
 
+/* Online autotuning of the persistent Gauss-Seidel launches (no counterpart in the
+ * reference, whose So-bogus solve has no launch shape; this is for callers that step
+ * one kind of scene many times). Enabled, each sweep launch runs on the cost model's
+ * workgroup count times 2^e, e in [-3, 3]; per class of launch (dimension, precision
+ * phase, ensemble or not, colours, contacts in half-octaves) the tuner times launches
+ * at neighbouring e with HIP events, climbs while the device time per sweep falls by
+ * more than 3 %, then keeps the fastest. Launches with fewer than 16 sweeps are not
+ * timed. Results do not depend on it: every width gives the same iterates. Off by
+ * default; the table lives as long as the context. */
+int scx_set_autotune(scx_ctx* ctx, int32_t enable);
+/* The tuner's table, one entry per class: key (bits 0-7 contacts in half-octaves, 8 3D,
+ * 9 FP64 phase, 10 ensemble, 11-15 barrier intervals per sweep / 2), the exponent in use,
+ * its workgroups at the last launch, its least time per sweep in microseconds (< 0: not
+ * timed yet) and whether the class has settled. Up to cap entries; count = all. */
+int scx_get_autotune(scx_ctx* ctx, uint32_t cap, uint32_t* keys, int32_t* exponent, int32_t* blocks,
+                     float* us_per_sweep, int32_t* settled, uint32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

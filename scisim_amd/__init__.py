@@ -35,7 +35,7 @@ EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_po
            "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
            "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
            "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
-           "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats"]
+           "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats", "scx_set_autotune", "scx_get_autotune"]
 
 MLP_PARAMS = 19075  # SCX_MLP_PARAMS
 WARM_LEARNED = 4
@@ -110,6 +110,8 @@ def lib():
         L.scx_unconstrained_flow.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint32, _pd, _pd, _d, _pd, _pd, _pd, _pd]
         L.scx_set_timing.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_timing.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p), _pd, _pu32]
+        L.scx_set_autotune.argtypes = [_vp, ctypes.c_int32]
+        L.scx_get_autotune.argtypes = [_vp, ctypes.c_uint32, _pu32, _pi32, _pi32, _pf, _pi32, _pu32]
         L.scx_set_warm_model.argtypes = [_vp, _pf, ctypes.c_uint32]
         L.scx_get_warm_model.argtypes = [_vp, _pf, ctypes.c_uint32]
         L.scx_get_samples.argtypes = [_vp, ctypes.c_uint32, _pf, _pf, _pu32]
@@ -311,6 +313,26 @@ class Context:
         ms = np.zeros(32)
         self._check(lib().scx_get_timing(self._h, 32, names, _p(ms), ctypes.byref(cnt)), "scx_get_timing")
         return [(names[i].decode(), float(ms[i])) for i in range(min(cnt.value, 32))]
+
+    def set_autotune(self, on):
+        """Online autotuning of the sweep launch width (results do not depend on it)."""
+        self._check(lib().scx_set_autotune(self._h, 1 if on else 0), "scx_set_autotune")
+
+    def autotune_table(self):
+        """The tuner's classes: list of dicts with key, exponent, blocks, us_per_sweep, settled."""
+        cnt = ctypes.c_uint32(0)
+        self._check(lib().scx_get_autotune(self._h, 0, None, None, None, None, None, ctypes.byref(cnt)),
+                    "scx_get_autotune")
+        k = cnt.value
+        keys = np.zeros(max(k, 1), dtype=np.uint32)
+        ex, bl, st = (np.zeros(max(k, 1), dtype=np.int32) for _ in range(3))
+        us = np.zeros(max(k, 1), dtype=np.float32)
+        self._check(lib().scx_get_autotune(self._h, k, _p(keys, _pu32), _p(ex, _pi32), _p(bl, _pi32), _p(us, _pf),
+                                           _p(st, _pi32), ctypes.byref(cnt)), "scx_get_autotune")
+        return [{"key": int(keys[i]), "contacts_half_octave": int(keys[i] & 0xFF), "dim": 3 if keys[i] >> 8 & 1 else 2,
+                 "phase": "fp64" if keys[i] >> 9 & 1 else "fp32", "ensemble": bool(keys[i] >> 10 & 1),
+                 "exponent": int(ex[i]), "blocks": int(bl[i]), "us_per_sweep": float(us[i]),
+                 "settled": bool(st[i])} for i in range(k)]
 
     # learned warm start
     def set_warm_model(self, params):

@@ -18,7 +18,7 @@ ARCH = os.environ.get("SCX_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code; FMA contraction is chosen
 # per file with '#pragma clang fp contract' (off wherever results must round like the
-# FP64 CPU restatement, fast in the sweeps).
+# FP64 CPU restatement, per expression in the sweeps).
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-value",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              ]

@@ -524,6 +524,10 @@ void scx_destroy(scx_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   free_all(c);
   for (hipEvent_t e : c->events) hipEventDestroy(e);
+  for (auto& P : c->tuner.pend)
+    for (hipEvent_t e : P.ev)
+      if (e) hipEventDestroy(e);
+  if (c->tuner.copied) hipEventDestroy(c->tuner.copied);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1229,6 +1233,43 @@ int scx_unconstrained_flow(scx_ctx* c, int32_t integrator, uint32_t n, const dou
 int scx_set_timing(scx_ctx* c, int32_t enable) {
   if (!c) return SCX_E_ARG;
   c->timing = enable != 0;
+  return SCX_OK;
+}
+
+int scx_set_autotune(scx_ctx* c, int32_t enable) {
+  if (!c) return SCX_E_ARG;
+  HIPC(hipSetDevice(c->device));
+  scx::SweepTuner& T = c->tuner;
+  if (enable && !T.copied) {
+    for (auto& P : T.pend)
+      for (hipEvent_t& e : P.ev) HIPC(hipEventCreate(&e));
+    HIPC(hipEventCreate(&T.copied));
+  }
+  T.enabled = enable != 0;
+  if (!T.enabled) {
+    T.pend[0].live = T.pend[1].live = false;
+    T.counters_live = false;
+  }
+  return SCX_OK;
+}
+
+int scx_get_autotune(scx_ctx* c, uint32_t cap, uint32_t* keys, int32_t* exponent, int32_t* blocks,
+                     float* us_per_sweep, int32_t* settled, uint32_t* count) {
+  if (!c) return SCX_E_ARG;
+  const scx::SweepTuner& T = c->tuner;
+  uint32_t k = 0;
+  for (const auto& kv : T.table) {
+    if (k < cap) {
+      const scx::SweepTuner::Entry& E = kv.second;
+      if (keys) keys[k] = kv.first;
+      if (exponent) exponent[k] = E.cur;
+      if (blocks) blocks[k] = scx::SweepTuner::blocks(E.model, E.cur, E.max_blocks);
+      if (us_per_sweep) us_per_sweep[k] = E.us[E.cur - scx::SweepTuner::kMinExp];
+      if (settled) settled[k] = E.dir == 0 ? 1 : 0;
+    }
+    ++k;
+  }
+  if (count) *count = k;
   return SCX_OK;
 }
 

@@ -12,11 +12,20 @@
 // together. The sweep works in primal-velocity form: the body velocities
 // v = v0 + M^-1 H^T r are kept current, so a contact's pre-impulse velocity is
 // H v + w and no Delassus off-diagonal block is ever formed.
-#pragma clang fp contract(fast)
+// FMA contraction per source expression only: the backend forms no further fusions, so
+// the same expression rounds the same way in every inlined copy (a contact updated as
+// the first of its thread's colour pass or as a later one, by the main loop or the tail),
+// which keeps the iterates independent of the launch width.
+#pragma clang fp contract(on)
 
 #include "scx_device.h"
 #include "scx_flow.h"
 #include "scx_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <type_traits>
 
 namespace scx {
 
@@ -988,6 +997,148 @@ __global__ void __launch_bounds__(kBlock) k_impulses_out(uint32_t C, int dim, co
 // solve stays below that.
 constexpr size_t kGSLdsMax = 124 * 1024;
 
+// ---------------------------------------------------------------------------
+// Online autotuning of the sweep launch width (SweepTuner in scx_internal.h).
+// ---------------------------------------------------------------------------
+constexpr int kTuneWords = 100;  // host_u32[100..101]: the step's sweeps and FP32 sweeps
+
+static int tuned_blocks(int model, int e, int max_blocks) { return SweepTuner::blocks(model, e, max_blocks); }
+
+static float tuned_us(const SweepTuner::Entry& E, int e) { return E.us[e - SweepTuner::kMinExp]; }
+
+// e is worth a launch after `from`: in range and a different workgroup count.
+static bool tune_useful(const SweepTuner::Entry& E, int e, int from) {
+  return e >= SweepTuner::kMinExp && e <= SweepTuner::kMaxExp &&
+         tuned_blocks(E.model, e, E.max_blocks) != tuned_blocks(E.model, from, E.max_blocks);
+}
+
+static void tune_settle(SweepTuner::Entry& E) {
+  int best = E.start;
+  float bu = -1.f;
+  for (int e = SweepTuner::kMinExp; e <= SweepTuner::kMaxExp; ++e) {
+    const float u = tuned_us(E, e);
+    if (u >= 0.f && (bu < 0.f || u < bu)) {
+      bu = u;
+      best = e;
+    }
+  }
+  E.cur = (int8_t)best;
+  E.dir = 0;
+}
+
+// The next exponent after a timed launch at E.cur: up from the start while each doubling
+// saves more than the margin, else down from the start likewise, then the fastest seen.
+static void tune_climb(SweepTuner::Entry& E) {
+  constexpr float kMargin = 0.97f;
+  const int x = E.cur;
+  if (x == E.start) {
+    if (tune_useful(E, x + 1, x)) {
+      E.cur = (int8_t)(x + 1);
+      E.dir = 1;
+    } else if (tune_useful(E, x - 1, x)) {
+      E.cur = (int8_t)(x - 1);
+      E.dir = -1;
+    } else {
+      tune_settle(E);
+    }
+    return;
+  }
+  const int prev = x - E.dir;
+  if (tuned_us(E, prev) >= 0.f && tuned_us(E, x) < kMargin * tuned_us(E, prev)) {
+    if (tune_useful(E, x + E.dir, x))
+      E.cur = (int8_t)(x + E.dir);
+    else
+      tune_settle(E);
+    return;
+  }
+  if (E.dir > 0 && prev == E.start && tuned_us(E, E.start - 1) < 0.f && tune_useful(E, E.start - 1, E.start)) {
+    E.cur = (int8_t)(E.start - 1);
+    E.dir = -1;
+    return;
+  }
+  tune_settle(E);
+}
+
+// Folds the timed launches of the previous solve into the table (their events and the
+// copy of the sweep counters completed at the stream synchronisations since).
+static void tune_resolve(scx_ctx* c) {
+  SweepTuner& T = c->tuner;
+  if (!T.counters_live) return;
+  T.counters_live = false;
+  const bool ok = hipEventSynchronize(T.copied) == hipSuccess;
+  const uint32_t total = c->d.host_u32[kTuneWords], fp32 = c->d.host_u32[kTuneWords + 1];
+  for (int ph = 0; ph < 2; ++ph) {
+    SweepTuner::Pending& P = T.pend[ph];
+    if (!P.live) continue;
+    P.live = false;
+    float ms = 0.f;
+    if (!ok || hipEventElapsedTime(&ms, P.ev[0], P.ev[1]) != hipSuccess) continue;
+    auto it = T.table.find(P.key);
+    if (it == T.table.end()) continue;
+    SweepTuner::Entry& E = it->second;
+    const uint32_t n = (ph == 0) ? fp32 : total - fp32;
+    const int i = P.exp - SweepTuner::kMinExp;
+    if (n < SweepTuner::kMinSweeps) {
+      if (E.dir != 0 && P.exp == E.cur && ++E.misses[i] >= SweepTuner::kMaxMisses) tune_settle(E);
+      continue;
+    }
+    const float u = 1000.f * ms / (float)n;
+    E.us[i] = (E.us[i] < 0.f) ? u : std::min(E.us[i], u);
+    if (E.dir != 0 && P.exp == E.cur) tune_climb(E);
+  }
+}
+
+// Class of a sweep launch: bits 0-7 contacts in half-octaves, 8 3D, 9 FP64 phase,
+// 10 ensemble, 11-15 barrier intervals per sweep in pairs.
+static uint32_t tune_key(int D, int phase, bool ens, uint32_t C, int nintervals) {
+  const uint32_t cb = (uint32_t)std::min(255.0, std::floor(2.0 * std::log2((double)std::max(1u, C))));
+  const uint32_t wb = (uint32_t)std::min(31, (nintervals + 1) / 2);
+  return cb | ((uint32_t)(D == 3) << 8) | ((uint32_t)phase << 9) | ((uint32_t)ens << 10) | (wb << 11);
+}
+
+// Workgroups of the next launch of a class; a new class starts from the settled exponent
+// of the nearest class of the same dimension, phase and kind.
+static int tune_pick(scx_ctx* c, int phase, uint32_t key, int model, int max_blocks) {
+  SweepTuner& T = c->tuner;
+  auto it = T.table.find(key);
+  if (it == T.table.end()) {
+    SweepTuner::Entry E;
+    for (float& u : E.us) u = -1.f;
+    for (uint8_t& m : E.misses) m = 0;
+    int best_d = 1 << 30;
+    for (const auto& kv : T.table) {
+      if (((kv.first >> 8) & 7u) != ((key >> 8) & 7u) || kv.second.dir != 0) continue;
+      const int d = std::abs((int)(kv.first & 0xFFu) - (int)(key & 0xFFu)) +
+                    std::abs((int)(kv.first >> 11) - (int)(key >> 11));
+      if (d < best_d) {
+        best_d = d;
+        E.start = E.cur = kv.second.cur;
+      }
+    }
+    it = T.table.emplace(key, E).first;
+  }
+  SweepTuner::Entry& E = it->second;
+  E.model = model;
+  E.max_blocks = max_blocks;
+  ++E.launches;
+  SweepTuner::Pending& P = T.pend[phase];
+  P.live = true;
+  P.key = key;
+  P.exp = E.cur;
+  return tuned_blocks(model, E.cur, max_blocks);
+}
+
+// After a solve with timed launches: the sweep counters for their measurement.
+static hipError_t tune_after_solve(scx_ctx* c) {
+  SweepTuner& T = c->tuner;
+  if (!T.enabled || !(T.pend[0].live || T.pend[1].live)) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(c->d.host_u32 + kTuneWords, &c->d.ctr->sweeps, 2 * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(T.copied, c->stream);
+  T.counters_live = e == hipSuccess;
+  return e;
+}
+
 template <int D, typename Real>
 static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint32_t* col_off) {
   // Tail: the suffix of colours no wider than one workgroup, swept by one workgroup (no
@@ -1007,11 +1158,18 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
     if (col_off[b + 1] > col_off[b]) widths[nw++] = col_off[b + 1] - col_off[b];
   if (tail < ncol) widths[nw++] = 1;
   widths[nw++] = A.C;
-  const int want = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks);
+  const int model = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks);
+  const size_t lds = LdsLayout<D, Real>::bytes(A.C);
+  const bool lds_path = std::min(max_blocks, model) <= 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap &&
+                        !A.cscene;
+  // the autotuner picks among widths around the model's (not for the LDS solve)
+  const int phase = std::is_same<Real, double>::value ? 1 : 0;
+  const bool tune = c->tuner.enabled && c->force_blocks <= 0 && !lds_path;
+  const int want = tune ? tune_pick(c, phase, tune_key(D, phase, A.cscene != nullptr, A.C, nw), model, max_blocks)
+                        : model;
   const int blocks = std::max(1, std::min(max_blocks, want));
   A.tail_col = (blocks > 1) ? (uint32_t)tail : (uint32_t)kMaxColours;
-  const size_t lds = LdsLayout<D, Real>::bytes(A.C);
-  if (blocks == 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap && !A.cscene) {
+  if (lds_path) {
     // the whole solve in one workgroup's LDS
     hipError_t e = hipFuncSetAttribute((const void*)k_gs_lds<D, Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
@@ -1020,7 +1178,11 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
     return hipGetLastError();
   }
   void* args[] = {&A};
-  return hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
+  hipError_t e = tune ? hipEventRecord(c->tuner.pend[phase].ev[0], c->stream) : hipSuccess;
+  if (e != hipSuccess) return e;
+  e = hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
+  if (e == hipSuccess && tune) e = hipEventRecord(c->tuner.pend[phase].ev[1], c->stream);
+  return e;
 }
 
 template <int D, typename Real>
@@ -1046,6 +1208,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_scene(uint32_t C, const uint
 template <int D>
 static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, const scx_step_params* p) {
   hipStream_t s = c->stream;
+  if (c->tuner.enabled) tune_resolve(c);
   const bool fp32 = p->precision == SCX_PRECISION_FP32_POLISH;
   const bool warm = p->warm_start != SCX_WARM_NONE;
   const uint32_t nb = (c->n + kBlock - 1) / kBlock;
@@ -1092,7 +1255,9 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
     B.scene_sweeps = c->d.scene_sweeps;
     B.nscenes = c->nscenes;
   }
-  return coop_gs<D, double>(c, B, c->gs_blocks, col_off);
+  e = coop_gs<D, double>(c, B, c->gs_blocks, col_off);
+  if (e != hipSuccess) return e;
+  return tune_after_solve(c);
 }
 
 hipError_t launch_solve(scx_ctx* c, uint32_t C, const uint32_t* col_off, const scx_step_params* p) {

@@ -52,3 +52,13 @@ def gs_eval_bytes(dim, fp32):
 
 def gs_launch_bytes(dim, ncontacts, sweeps, evals, fp32):
     return ncontacts * (sweeps * gs_sweep_bytes(dim, fp32) + evals * gs_eval_bytes(dim, fp32))
+
+
+def broadphase_bytes(dim, n, npairs):
+    """The broad phase of one step as a whole (predictor with keys, binning, cell ranges,
+    neighbour sweep with the exact filter, ordered pair output), counted independently of
+    how it is implemented: per body 52 + 48 + 16 + 48 + 8 = 172 B in 2D and
+    72 + 48 + 12 + 68 + 8 = 208 B in 3D (FP64 state read once, FP32 boxes and radii,
+    three 8-byte key/id radix passes, cell ranges, per-owner count and offset), plus 8 B
+    per emitted pair."""
+    return (172 if dim == 2 else 208) * n + 8 * npairs
