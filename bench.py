@@ -15,8 +15,9 @@ Workloads (synthetic, seeded):
   cfg4: an ensemble of 512 independent poured piles stepped together (scx_set_scenes):
         body counts log-uniform in [500, 20000], radii U[0.03, 0.07], mu U[0.1, 0.8] per
         scene, Verlet, dt = 0.005; settled under the transported cache, then the learned
-        warm start trained as in cfg3. Every scene completes each step together, so the
-        per-scene step latency is the step's wall time (p50/p99 over the timed steps).
+        warm start trained as in cfg3. A scene is frozen once it meets the tolerance, so its
+        latency in a step is the time outside the sweeps plus its share of the sweep time
+        (p50/p99 over the scenes of the last step), next to the wall time of whole steps.
 
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
 narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final

@@ -64,6 +64,9 @@ def build_cpp_tests():
         _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
               os.path.join(src_dir, name + ".cpp"), "-o", os.path.join(out_dir, name), "-L", HERE, "-lscx",
               "-Wl,-rpath," + HERE])
+    # the autotuner's search, host code only (tests/test_tuner_search.py)
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", CSRC, os.path.join(src_dir, "tuner_tests.cpp"), "-o",
+          os.path.join(out_dir, "tuner_tests")])
 
 
 def build():

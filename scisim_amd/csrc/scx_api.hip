@@ -1258,13 +1258,13 @@ int scx_get_autotune(scx_ctx* c, uint32_t cap, uint32_t* keys, int32_t* exponent
   if (!c) return SCX_E_ARG;
   const scx::SweepTuner& T = c->tuner;
   uint32_t k = 0;
-  for (const auto& kv : T.table) {
+  for (const auto& kv : T.table.entries()) {
     if (k < cap) {
-      const scx::SweepTuner::Entry& E = kv.second;
+      const scx::TuneEntry& E = kv.second;
       if (keys) keys[k] = kv.first;
       if (exponent) exponent[k] = E.cur;
-      if (blocks) blocks[k] = scx::SweepTuner::blocks(E.model, E.cur, E.max_blocks);
-      if (us_per_sweep) us_per_sweep[k] = E.us[E.cur - scx::SweepTuner::kMinExp];
+      if (blocks) blocks[k] = scx::TuneTable::blocks(E.model, E.cur, E.max_blocks);
+      if (us_per_sweep) us_per_sweep[k] = E.at(E.cur);
       if (settled) settled[k] = E.dir == 0 ? 1 : 0;
     }
     ++k;

@@ -5,11 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/scx.h"
+#include "scx_tune.h"
 
 namespace scx {
 
@@ -78,41 +78,19 @@ struct ContactSoA {
   Real* r;      // 3 per contact
 };
 
-// Online autotuner of the persistent sweep launch width (scx_set_autotune). A sweep
-// launch runs on the cost model's workgroup count times 2^e. Per workload class
-// (dimension, precision phase, ensemble or not, colours, contacts in half-octaves) the
-// tuner times launches at neighbouring exponents with HIP events, climbing from the
-// class's start exponent while the device time per sweep falls by more than the noise
-// margin, and then keeps the fastest. A new class starts from the settled exponent of
-// the nearest class of its kind. Every width gives the same iterates (the contacts of one
-// colour are independent and the residual is a maximum), so tuning changes only time.
+// Online autotuner of the persistent sweep launch width (scx_set_autotune): the search
+// (TuneTable, scx_tune.h) plus the HIP events that time the launches. Every width gives
+// the same iterates (the contacts of one colour are independent and the residual is a
+// maximum), so tuning changes only time.
 struct SweepTuner {
-  static constexpr int kMinExp = -3, kMaxExp = 3, kExps = kMaxExp - kMinExp + 1;
-  static constexpr uint32_t kMinSweeps = 16;  // launches with fewer sweeps are not timed
-  static constexpr int kMaxMisses = 3;        // untimed launches before a class settles
-  struct Entry {
-    float us[kExps];                // least device microseconds per sweep seen at e (< 0: none)
-    uint8_t misses[kExps];
-    int8_t cur = 0;                 // exponent of the next launch
-    int8_t start = 0;
-    int8_t dir = 1;                 // +1 climbing up, -1 climbing down, 0 settled
-    int model = 1, max_blocks = 1;  // of the last launch (bounds the useful exponents)
-    uint32_t launches = 0;
-  };
   struct Pending {
     bool live = false;
     uint32_t key = 0;
     int exp = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
   };
-  // workgroups of exponent e around the model's pick, within [1, max_blocks]
-  static int blocks(int model, int e, int max_blocks) {
-    long long b = model;
-    b = (e >= 0) ? (b << e) : (b >> -e);
-    return (int)(b < 1 ? 1 : (b > max_blocks ? max_blocks : b));
-  }
   bool enabled = false;
-  std::map<uint32_t, Entry> table;
+  TuneTable table;
   Pending pend[2];              // last launch of the FP32 (0) and FP64 (1) sweep phases
   hipEvent_t copied = nullptr;  // after the copy of the step's sweep counters
   bool counters_live = false;

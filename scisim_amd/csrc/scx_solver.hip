@@ -1002,63 +1002,6 @@ constexpr size_t kGSLdsMax = 124 * 1024;
 // ---------------------------------------------------------------------------
 constexpr int kTuneWords = 100;  // host_u32[100..101]: the step's sweeps and FP32 sweeps
 
-static int tuned_blocks(int model, int e, int max_blocks) { return SweepTuner::blocks(model, e, max_blocks); }
-
-static float tuned_us(const SweepTuner::Entry& E, int e) { return E.us[e - SweepTuner::kMinExp]; }
-
-// e is worth a launch after `from`: in range and a different workgroup count.
-static bool tune_useful(const SweepTuner::Entry& E, int e, int from) {
-  return e >= SweepTuner::kMinExp && e <= SweepTuner::kMaxExp &&
-         tuned_blocks(E.model, e, E.max_blocks) != tuned_blocks(E.model, from, E.max_blocks);
-}
-
-static void tune_settle(SweepTuner::Entry& E) {
-  int best = E.start;
-  float bu = -1.f;
-  for (int e = SweepTuner::kMinExp; e <= SweepTuner::kMaxExp; ++e) {
-    const float u = tuned_us(E, e);
-    if (u >= 0.f && (bu < 0.f || u < bu)) {
-      bu = u;
-      best = e;
-    }
-  }
-  E.cur = (int8_t)best;
-  E.dir = 0;
-}
-
-// The next exponent after a timed launch at E.cur: up from the start while each doubling
-// saves more than the margin, else down from the start likewise, then the fastest seen.
-static void tune_climb(SweepTuner::Entry& E) {
-  constexpr float kMargin = 0.97f;
-  const int x = E.cur;
-  if (x == E.start) {
-    if (tune_useful(E, x + 1, x)) {
-      E.cur = (int8_t)(x + 1);
-      E.dir = 1;
-    } else if (tune_useful(E, x - 1, x)) {
-      E.cur = (int8_t)(x - 1);
-      E.dir = -1;
-    } else {
-      tune_settle(E);
-    }
-    return;
-  }
-  const int prev = x - E.dir;
-  if (tuned_us(E, prev) >= 0.f && tuned_us(E, x) < kMargin * tuned_us(E, prev)) {
-    if (tune_useful(E, x + E.dir, x))
-      E.cur = (int8_t)(x + E.dir);
-    else
-      tune_settle(E);
-    return;
-  }
-  if (E.dir > 0 && prev == E.start && tuned_us(E, E.start - 1) < 0.f && tune_useful(E, E.start - 1, E.start)) {
-    E.cur = (int8_t)(E.start - 1);
-    E.dir = -1;
-    return;
-  }
-  tune_settle(E);
-}
-
 // Folds the timed launches of the previous solve into the table (their events and the
 // copy of the sweep counters completed at the stream synchronisations since).
 static void tune_resolve(scx_ctx* c) {
@@ -1073,18 +1016,7 @@ static void tune_resolve(scx_ctx* c) {
     P.live = false;
     float ms = 0.f;
     if (!ok || hipEventElapsedTime(&ms, P.ev[0], P.ev[1]) != hipSuccess) continue;
-    auto it = T.table.find(P.key);
-    if (it == T.table.end()) continue;
-    SweepTuner::Entry& E = it->second;
-    const uint32_t n = (ph == 0) ? fp32 : total - fp32;
-    const int i = P.exp - SweepTuner::kMinExp;
-    if (n < SweepTuner::kMinSweeps) {
-      if (E.dir != 0 && P.exp == E.cur && ++E.misses[i] >= SweepTuner::kMaxMisses) tune_settle(E);
-      continue;
-    }
-    const float u = 1000.f * ms / (float)n;
-    E.us[i] = (E.us[i] < 0.f) ? u : std::min(E.us[i], u);
-    if (E.dir != 0 && P.exp == E.cur) tune_climb(E);
+    T.table.record(P.key, P.exp, ms, (ph == 0) ? fp32 : total - fp32);
   }
 }
 
@@ -1096,36 +1028,15 @@ static uint32_t tune_key(int D, int phase, bool ens, uint32_t C, int nintervals)
   return cb | ((uint32_t)(D == 3) << 8) | ((uint32_t)phase << 9) | ((uint32_t)ens << 10) | (wb << 11);
 }
 
-// Workgroups of the next launch of a class; a new class starts from the settled exponent
-// of the nearest class of the same dimension, phase and kind.
+// Workgroups of the next launch of a class (the exponent is remembered for its timing).
 static int tune_pick(scx_ctx* c, int phase, uint32_t key, int model, int max_blocks) {
   SweepTuner& T = c->tuner;
-  auto it = T.table.find(key);
-  if (it == T.table.end()) {
-    SweepTuner::Entry E;
-    for (float& u : E.us) u = -1.f;
-    for (uint8_t& m : E.misses) m = 0;
-    int best_d = 1 << 30;
-    for (const auto& kv : T.table) {
-      if (((kv.first >> 8) & 7u) != ((key >> 8) & 7u) || kv.second.dir != 0) continue;
-      const int d = std::abs((int)(kv.first & 0xFFu) - (int)(key & 0xFFu)) +
-                    std::abs((int)(kv.first >> 11) - (int)(key >> 11));
-      if (d < best_d) {
-        best_d = d;
-        E.start = E.cur = kv.second.cur;
-      }
-    }
-    it = T.table.emplace(key, E).first;
-  }
-  SweepTuner::Entry& E = it->second;
-  E.model = model;
-  E.max_blocks = max_blocks;
-  ++E.launches;
+  const int e = T.table.pick(key, model, max_blocks);
   SweepTuner::Pending& P = T.pend[phase];
   P.live = true;
   P.key = key;
-  P.exp = E.cur;
-  return tuned_blocks(model, E.cur, max_blocks);
+  P.exp = e;
+  return TuneTable::blocks(model, e, max_blocks);
 }
 
 // After a solve with timed launches: the sweep counters for their measurement.
