@@ -26,7 +26,9 @@ settings): 50000 for cfg1-cfg3, so that every step reaches the tolerance; 5000 f
 ensemble, whose steps would otherwise run for minutes (config.steps_at_tolerance says how
 many did). value = bodies * steps / wall time of the
 timed steps. The dominant kernel's roofline uses algorithmic bytes per launch
-(scisim_amd/roofline.py) over its average duration from HIP events on the library's stream. The CPU
+(scisim_amd/roofline.py) over its average duration from HIP events on the library's stream;
+broadphase_roofline does the same for the broad phase as a whole (predictor, binning, both
+neighbour sweeps) with the implementation-independent per-body and per-pair bytes. The CPU
 baseline is the FP64 oracle (a restatement of the reference path) timed on a bounded
 prefix of the same workload on the host.
 """
