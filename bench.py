@@ -85,8 +85,10 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log, compare=True):
     tp = scx.step_params(precision=precision, **dict(prm, warm_start=scx.WARM_TRANSPORT))
     Xs, Ts = [], []
     frac = 0.25 if s["r"].size <= 300000 else 0.05  # samples kept per step (host memory)
+    t0 = time.perf_counter()
     for k in range(args.collect):
-        ctx.step(tp)
+        st = ctx.step(tp)
+        log("collect", k, "contacts", st["ncontacts"], "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
         X, T = ctx.samples()
         keep = rng.random(X.shape[0]) < frac
         Xs.append(X[keep])
