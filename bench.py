@@ -216,8 +216,8 @@ def main():
     if args.steps is None:
         args.steps = 100 if args.config == "cfg1" else 20
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (requested --gpus %d); multi-GPU scaling is not part of "
-              "it" % args.gpus, file=sys.stderr)
+        print("bench.py: the benchmark runs on one GPU only (requested --gpus %d); the library has no multi-GPU "
+              "path" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration > 0:
         import torch
