@@ -89,3 +89,25 @@ def test_poured_pile_rests_on_its_supports():
     # a pile, not a gas: about four contacts per sphere or more before settling
     contacts = (touching.sum() + (q[:, 1] - r <= 1e-7).sum()) / len(r)
     assert contacts > 3.5
+
+
+def test_ensemble_generator_offsets_and_ranges():
+    """ensemble_piles: seeded, offsets consistent with the concatenated arrays, body counts,
+    radii and friction coefficients in their ranges, five planes per scene, unit-density
+    masses and sphere inertias, and the same result with one worker thread or several."""
+    from scisim_amd import scenes
+    kw = dict(nscenes=6, nmin=100, nmax=400, rmin=0.03, rmax=0.07, mu_lo=0.1, mu_hi=0.8, seed=5, workers=3)
+    s, bo, po, mus, counts = scenes.ensemble_piles(**kw)
+    s2, bo2, _, mus2, _ = scenes.ensemble_piles(**dict(kw, workers=1))
+    assert np.array_equal(s["q"], s2["q"]) and np.array_equal(bo, bo2) and np.array_equal(mus, mus2)
+    assert bo[0] == 0 and np.array_equal(np.diff(bo.astype(np.int64)), counts)
+    assert np.all((counts >= 100) & (counts <= 400))
+    n = int(bo[-1])
+    assert s["q"].size == 3 * n and s["r"].size == n and s["m"].size == n and s["inertia"].size == n
+    assert np.all((s["r"] >= 0.03) & (s["r"] <= 0.07))
+    assert np.all((mus >= 0.1) & (mus <= 0.8))
+    assert np.array_equal(po, np.arange(7, dtype=np.uint32) * 5)
+    assert np.asarray(s["px"]).reshape(-1, 3).shape[0] == 30
+    # masses and inertias of unit-density spheres
+    assert np.allclose(s["m"], 4.0 / 3.0 * np.pi * s["r"] ** 3, rtol=1e-12)
+    assert np.allclose(s["inertia"], 0.4 * s["m"] * s["r"] ** 2, rtol=1e-12)
