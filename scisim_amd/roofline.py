@@ -30,24 +30,27 @@ def pairs_launch_bytes(dim, n, ncells, nentries, npairs, write_pass):
     return b
 
 
+def _record_bytes(dim):
+    """Contact record with FP32 working arrays: 2D 32 B (body ids 8, normal 8, Delassus
+    diagonal 4, forcing 8, mu 4); 3D 64 B (ids 8, n and s 24, arms 8, W_n and W_t 8,
+    forcing 12, mu 4)."""
+    return 32 if dim == 2 else 64
+
+
 def gs_sweep_bytes(dim, fp32):
-    """Per contact per Gauss-Seidel sweep: the contact record (body ids, frame, Delassus
-    diagonal, mu, forcing), the impulse read + write and both bodies' velocities (dof
-    components) read + write."""
-    w = 4 if fp32 else 8
+    """Per contact per Gauss-Seidel sweep: the record, the impulse read and written
+    (2 x 4 D) and both bodies' velocity deltas read and written (2 x 2 x 4 dof, dof = 2 in
+    2D, 6 in 3D): 2D 80 B, 3D 184 B. An FP64 sweep counts twice that."""
     dof = 2 if dim == 2 else 6
-    if dim == 2:
-        rec = 8 + 2 * w + w + w + w  # ids, n, W, mu, forcing
-    else:
-        rec = 8 + 6 * w + 2 * w + 2 * w + w + w  # ids, n, s, arms, W_n, W_t, mu, forcing
-    return rec + 2 * dim * w + 2 * 2 * dof * w
+    b = _record_bytes(dim) + 2 * 4 * dim + 2 * 2 * 4 * dof
+    return b if fp32 else 2 * b
 
 
 def gs_eval_bytes(dim, fp32):
-    """Residual evaluation per contact: record and impulse read, both bodies read."""
-    w = 4 if fp32 else 8
+    """Residual evaluation per contact (either phase): the record, the impulse and both
+    bodies read in FP64 (record + 8 D + 2 x 8 dof): 2D 80 B, 3D 184 B."""
     dof = 2 if dim == 2 else 6
-    return gs_sweep_bytes(dim, fp32) - dim * w - 2 * dof * w
+    return _record_bytes(dim) + 8 * dim + 2 * 8 * dof
 
 
 def gs_launch_bytes(dim, ncontacts, sweeps, evals, fp32):

@@ -1,0 +1,46 @@
+"""The algorithmic byte counts behind the bench's roofline figures
+(scisim_amd/roofline.py), checked against the per-unit figures derived field by field
+from the record layouts: a dropped or doubled term changes one of these numbers."""
+
+from scisim_amd import roofline as rl
+
+
+def test_sweep_bytes_per_contact():
+    # 2D FP32: record 32 (ids 8, n 8, W 4, forcing 8, mu 4), impulse read + write 2*2*4,
+    # both bodies' 2-component velocity deltas read + write 2*2*2*4
+    assert rl.gs_sweep_bytes(2, True) == 32 + 16 + 32 == 80
+    # 3D FP32: record 64 (ids 8, n and s 24, arms 8, W_n W_t 8, forcing 12, mu 4), impulse
+    # 2*3*4, both bodies' 6 dof 2*2*6*4
+    assert rl.gs_sweep_bytes(3, True) == 64 + 24 + 96 == 184
+    # an FP64 sweep counts twice the FP32 figure
+    assert rl.gs_sweep_bytes(2, False) == 160 and rl.gs_sweep_bytes(3, False) == 368
+
+
+def test_residual_bytes_per_contact():
+    # record, impulse and both bodies read in FP64: 32 + 8*2 + 2*8*2 and 64 + 8*3 + 2*8*6
+    assert rl.gs_eval_bytes(2, True) == rl.gs_eval_bytes(2, False) == 80
+    assert rl.gs_eval_bytes(3, True) == rl.gs_eval_bytes(3, False) == 184
+
+
+def test_launch_bytes_add_up():
+    C, S = 1000, 7
+    assert rl.gs_launch_bytes(2, C, S, S + 1, True) == C * (S * rl.gs_sweep_bytes(2, True) +
+                                                          (S + 1) * rl.gs_eval_bytes(2, True))
+    assert rl.gs_launch_bytes(3, 0, S, S + 1, False) == 0
+
+
+def test_pairs_bytes():
+    n, G, E, P = 1000, 500, 4000, 300
+    per_body = 4 + 16 + 40 + 4
+    assert rl.pairs_bytes_per_body(2) == per_body
+    assert rl.pairs_bytes_per_body(3) == 4 + 24 + 32 + 4
+    assert rl.pairs_launch_bytes(2, n, G, E, P, False) == n * per_body + 4 * (G + 1) + 4 * E
+    assert rl.pairs_launch_bytes(2, n, G, E, P, True) - rl.pairs_launch_bytes(2, n, G, E, P, False) == 8 * P
+
+
+def test_broadphase_bytes():
+    # per body: predictor 52/72, three 16-byte radix passes 48, cell ranges 16/12,
+    # neighbour sweep 48/68, ordered output 8; per pair 8
+    assert rl.broadphase_bytes(2, 1, 0) == 52 + 48 + 16 + 48 + 8 == 172
+    assert rl.broadphase_bytes(3, 1, 0) == 72 + 48 + 12 + 68 + 8 == 208
+    assert rl.broadphase_bytes(2, 10, 3) == 1720 + 24
