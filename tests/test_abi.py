@@ -6,10 +6,12 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import scisim_amd as scx
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "scx.h")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "scx.h")
 
 
 def declared_symbols():
@@ -111,3 +113,19 @@ def test_ensemble_generator_offsets_and_ranges():
     # masses and inertias of unit-density spheres
     assert np.allclose(s["m"], 4.0 / 3.0 * np.pi * s["r"] ** 3, rtol=1e-12)
     assert np.allclose(s["inertia"], 0.4 * s["m"] * s["r"] ** 2, rtol=1e-12)
+
+
+@pytest.mark.parametrize("lang,header", [("c", "scx.h"), ("c++", "scx.h"), ("c++", "scx_scisim.hpp")])
+def test_public_headers_compile_on_their_own(lang, header):
+    """The C ABI header is valid C99 and C++17 by itself, and the reference-shaped C++
+    layer compiles with nothing but its own includes (no HIP or torch headers needed)."""
+    import shutil
+    import subprocess
+    cc = "gcc" if lang == "c" else "g++"
+    if shutil.which(cc) is None:
+        pytest.skip(cc + " not found")
+    std = "-std=c99" if lang == "c" else "-std=c++17"
+    src = '#include "%s"\n' % header
+    p = subprocess.run([cc, std, "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), "-x", lang, "-"], input=src, text=True, capture_output=True)
+    assert p.returncode == 0, p.stderr
