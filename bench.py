@@ -1,36 +1,32 @@
 """Benchmark: body-steps/s of the per-step contact pipeline on one MI355X.
 
-Workloads (synthetic, seeded):
-  cfg1 (default): ball2d, 1,048,576 circles, radii U[0.02, 0.06], density 1, random
-        sequential packing at 55% area fraction in a box with 4 planes, mu = 0.3,
-        CoR = 0, dt = 0.005, symplectic Euler, FP32 sweeps + FP64 residual/polish,
-        residual tolerance 1e-6.
-  cfg2: rigidbody3d, 262,144 spheres, log-normal radii (median 0.05, sigma 0.25),
-        poured into a box with 5 planes (sequential deposition into a resting pile,
-        then untimed settling steps), mu = 0.4, CoR = 0, Verlet, dt = 0.005; plain
-        coloured Gauss-Seidel (cold start) unless --warm says otherwise.
-  cfg3: cfg2 with the learned warm start: samples of `--collect` steps, the perceptron
-        trained on them on the device (untimed), then the timed steps with it; the cold
-        and transported-cache warm starts are measured from the same state for comparison.
-  cfg4: an ensemble of 512 independent poured piles stepped together (scx_set_scenes):
-        body counts log-uniform in [500, 20000], radii U[0.03, 0.07], mu U[0.1, 0.8] per
-        scene, Verlet, dt = 0.005; settled under the transported cache, then the learned
-        warm start trained as in cfg3. A scene is frozen once it meets the tolerance, so its
-        latency in a step is the time outside the sweeps plus its share of the sweep time
-        (p50/p99 over the scenes of the last step), next to the wall time of whole steps.
+Default workload (cfg3, the configuration the headline target is quoted on): 262,144
+spheres with log-normal radii (median 0.05, sigma 0.25), poured into a box with 5 static
+planes and settled, 3D Coulomb mu = 0.4, CoR = 0, Verlet, dt = 0.005, FB residual
+tolerance 1e-6 (max over contacts of ||phi||^2; max ||phi|| is reported beside it),
+FP32 sweeps + FP64 residual/polish, stepped with the learned warm start (a 16-128-128-3
+BF16 perceptron trained on the device from samples of the transported-cache solver).
+From the same settled state the plain coloured Gauss-Seidel (cold start, as the
+reference's 3D geometric map) and the transported impulse cache are measured as well; the
+sweep and body-step ratios go into config.warm_start_comparison.
+
+Other workloads (--config): cfg1, 1,048,576 RSA circles (2D, mu 0.3, dt 0.005, 100
+steps); cfg2, the pile with the plain coloured GS as the stepping mode; cfg4, an ensemble
+of 512 independent piles (learned warm start, per-scene p50/p99 latency).
 
 A step is one GeometricImpactFrictionMap step over all bodies (predictor, broad and
-narrow phase, contact setup, colouring, coloured Gauss-Seidel to tolerance, final
-update), with the state resident in HBM. Sweep limit (the reference scenes' two
-settings): 50000 for cfg1-cfg3, so that every step reaches the tolerance; 5000 for the
-ensemble, whose steps would otherwise run for minutes (config.steps_at_tolerance says how
-many did). value = bodies * steps / wall time of the
-timed steps. The dominant kernel's roofline uses algorithmic bytes per launch
-(scisim_amd/roofline.py) over its average duration from HIP events on the library's stream;
-broadphase_roofline does the same for the broad phase as a whole (predictor, binning, both
-neighbour sweeps) with the implementation-independent per-body and per-pair bytes. The CPU
-baseline is the FP64 oracle (a restatement of the reference path) timed on a bounded
-prefix of the same workload on the host.
+narrow phase, contact setup, warm start, colouring, coloured Gauss-Seidel to tolerance,
+final update) with the state resident in HBM. --steps sets only the number of timed steps
+(each ends with a stream synchronisation); everything else (settling, sample collection,
+training, the comparison) is untimed and fixed by its own options. value = bodies * steps
+/ wall time of the timed steps. roofline: the dominant kernel by device time, its
+algorithmic bytes per launch (scisim_amd/roofline.py) over its mean launch duration from
+HIP events on the library's stream; traffic is null unless --traffic gives HBM bytes per
+launch measured by rocprofv3 --pmc in the same command. cpu_baseline: the FP64 oracle (the
+project's restatement of the reference path: natural-order Gauss-Seidel from a cold
+start, as the reference) on the state in the middle of the timed steps, on all host cores
+of this process (parallel residual evaluation and per-contact set-up; the sweep is
+sequential as in the reference) and on one thread, each on a bounded sample.
 """
 
 import argparse
@@ -43,6 +39,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
 
 def make_scene(cfg, seed):
     from scisim_amd import scenes
@@ -67,18 +64,26 @@ def make_scene(cfg, seed):
         s = scenes.spheres_pile(n=262144, rmed=0.05, sigma=0.25, seed=seed)
         prm = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, CoR=0.0, tol=1e-6, max_iters=50000, eval_every=1,
                    integrator=1)
-        desc = {"workload": "spheres3d_pile_262k" + ("_learned_warm_start" if cfg == "cfg3" else ""),
+        desc = {"workload": "spheres3d_pile_262k" + ("_learned_warm_start" if cfg == "cfg3" else "_plain_gs"),
                 "bodies": 262144, "dim": 3, "radii": "lognormal(0.05,0.25)",
                 "packing": "poured pile (sequential deposition), settled", "planes": 5, "mu": 0.4, "CoR": 0.0,
                 "dt": 0.005, "integrator": "verlet", "tol": 1e-6}
     return s, prm, desc
 
 
-def train_learned_warm_start(ctx, s, prm, precision, args, log, compare=True):
-    """cfg3: samples of `collect` steps under the transported cache (a quarter of the
-    contacts of each step, seeded), the perceptron trained on them on the device, then the
-    sweeps and step times of the cold, transported and learned warm starts over the same
-    `steps` steps from the same state. The context ends in that state with the model set."""
+def host_threads():
+    """Host threads this process may use: OMP_NUM_THREADS if set (the GPU hosts set it to
+    the job's CPU share), else the CPUs of its affinity mask."""
+    try:
+        env = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        env = 0
+    return env if env > 0 else len(os.sched_getaffinity(0))
+
+
+def collect_and_train(ctx, s, prm, precision, args, log):
+    """Samples of `collect` transported-cache steps (a fraction of the contacts of each
+    step, seeded), then the perceptron trained on them on the device."""
     import scisim_amd as scx
     from scisim_amd import warmstart
     rng = np.random.default_rng(args.seed)
@@ -88,29 +93,36 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log, compare=True):
     t0 = time.perf_counter()
     for k in range(args.collect):
         st = ctx.step(tp)
-        log("collect", k, "contacts", st["ncontacts"], "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
         X, T = ctx.samples()
         keep = rng.random(X.shape[0]) < frac
         Xs.append(X[keep])
         Ts.append(T[keep])
+        log("collect", k, "contacts", st["ncontacts"], "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
     X, T = np.concatenate(Xs), np.concatenate(Ts)
     ctx.set_warm_model(warmstart.init_params(seed=args.seed))
     t0 = time.perf_counter()
-    loss = ctx.train_warm_model(X, T, epochs=12, batch=2048, lr=1e-3, seed=args.seed)
+    loss = ctx.train_warm_model(X, T, epochs=args.epochs, batch=2048, lr=1e-3, seed=args.seed)
     train_s = time.perf_counter() - t0
     log("trained on", X.shape[0], "samples, loss", loss, "%.1fs" % train_s)
+    return {"training_samples": int(X.shape[0]), "collect_steps": args.collect, "epochs": args.epochs,
+            "train_s": train_s, "final_mse": loss}
+
+
+def compare_warm_starts(ctx, s, prm, precision, args, log):
+    """The plain coloured Gauss-Seidel (cold start), the transported cache and the learned
+    warm start, each for a fixed number of steps from the same state (a transported-cache
+    step first primes the cache of every mode). The context ends in that state."""
+    import scisim_amd as scx
+    tp = scx.step_params(precision=precision, **dict(prm, warm_start=scx.WARM_TRANSPORT))
     q, v, w = ctx.get_state()
-    out = {"training_samples": int(X.shape[0]), "collect_steps": args.collect, "epochs": 12, "train_s": train_s,
-           "final_mse": loss}
-    if not compare:
-        return out
-    # every mode starts from the same state, after one transported-cache step that primes
-    # the cache (a cold step leaves the cache empty, so the cold start runs last)
-    # (the cold start needs tens of thousands of sweeps per step: fewer steps of it)
-    for name, mode in (("transport", scx.WARM_TRANSPORT), ("learned", scx.WARM_LEARNED), ("none", scx.WARM_NONE)):
+    out = {}
+    plan = (("transport", scx.WARM_TRANSPORT, args.compare_steps), ("learned", scx.WARM_LEARNED, args.compare_steps),
+            ("cold", scx.WARM_NONE, args.cold_steps))
+    for name, mode, nsteps in plan:
+        if nsteps <= 0:
+            continue
         ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
         ctx.step(tp)
-        nsteps = args.steps if mode != scx.WARM_NONE else min(args.steps, 5)
         sw, ok = [], 0
         t0 = time.perf_counter()
         for k in range(nsteps):
@@ -119,21 +131,34 @@ def train_learned_warm_start(ctx, s, prm, precision, args, log, compare=True):
             ok += int(st["succeeded"])
             log(name, "step", k, "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
         dt = time.perf_counter() - t0
-        out[name] = {"steps": nsteps, "mean_sweeps": float(np.mean(sw)), "converged_steps": ok,
+        out[name] = {"steps": nsteps, "mean_sweeps": float(np.mean(sw)), "steps_at_tolerance": ok,
                      "body_steps_per_s": s["r"].size * nsteps / dt}
-        log(name, out[name], sw)
+        log(name, out[name])
     ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
     ctx.step(tp)
+    if "cold" in out and "learned" in out:
+        out["sweeps_cold_over_learned"] = out["cold"]["mean_sweeps"] / max(1e-9, out["learned"]["mean_sweeps"])
+        out["body_steps_learned_over_cold"] = (out["learned"]["body_steps_per_s"] /
+                                               out["cold"]["body_steps_per_s"])
+    if "transport" in out and "learned" in out:
+        out["sweeps_learned_over_transport"] = out["learned"]["mean_sweeps"] / max(1e-9,
+                                                                                  out["transport"]["mean_sweeps"])
+        out["body_steps_learned_over_transport"] = (out["learned"]["body_steps_per_s"] /
+                                                    out["transport"]["body_steps_per_s"])
+    if "cold" in out and "transport" in out:
+        out["sweeps_cold_over_transport"] = out["cold"]["mean_sweeps"] / max(1e-9, out["transport"]["mean_sweeps"])
     return out
 
 
-def cpu_baseline(s, state, prm, budget_s, sweeps_hint):
-    """The FP64 oracle (natural-order Gauss-Seidel, cold start as the reference's 3D map,
-    one thread) on the state in the middle of the timed steps: one step, its sweep limit
-    cut so that it fits the budget. If that step reaches the tolerance the rate is
-    measured; otherwise the step time is scaled linearly in the sweeps to the GPU's mean
-    sweep count (the natural and the coloured orders need about the same sweeps)."""
+def cpu_baseline(s, state, prm, budget_s, sweeps_hint, threads):
+    """The FP64 oracle (natural-order Gauss-Seidel, cold start as the reference's maps,
+    `threads` threads for its parallel parts) on the state in the middle of the timed
+    steps: one step, its sweep limit cut so that it fits the budget. If that step reaches
+    the tolerance the rate is measured; otherwise the step time is scaled linearly in the
+    sweeps to `sweeps_hint` (the GPU's cold sweep count from the same kind of state: the
+    natural and the coloured orders need about the same sweeps)."""
     import oracle
+    oracle.set_threads(threads)
     q, v, w = state
     D = s["dim"]
 
@@ -156,7 +181,7 @@ def cpu_baseline(s, state, prm, budget_s, sweeps_hint):
             done += int(body_off[k + 1] - body_off[k])
             ok += int(st["succeeded"])
             k += 1
-        return {"value": done / t_all, "unit": "body-steps/s", "cores": 1, "kind": "port",
+        return {"value": done / t_all, "unit": "body-steps/s", "cores": threads, "kind": "port",
                 "sample": "FP64 oracle, natural-order GS, one step of each of the first %d scenes alone from the "
                           "state in the middle of the timed steps (%d bodies, %d of them reached the tolerance), "
                           "%.1f s" % (k, done, ok, t_all)}
@@ -174,36 +199,53 @@ def cpu_baseline(s, state, prm, budget_s, sweeps_hint):
             m = int(min(prm["max_iters"], max(16, (budget_s - t1 - t16 - t1) / per)))
             t, st = one(0, n, 0, P, prm["mu"], m)
     if st["succeeded"]:
-        return {"value": n / t, "unit": "body-steps/s", "cores": 1, "kind": "port",
-                "sample": "FP64 oracle, natural-order GS from a cold start, one step from the state in the middle of "
-                          "the timed steps, to the tolerance in %d sweeps, %.1f s" % (st["sweeps"], t)}
+        return {"value": n / t, "unit": "body-steps/s", "cores": threads, "kind": "port",
+                "sample": "FP64 oracle on %d thread(s), natural-order GS from a cold start, one step from the state "
+                          "in the middle of the timed steps, to the tolerance in %d sweeps, %.1f s"
+                          % (threads, st["sweeps"], t)}
     per = max((t - t1) / max(m - 1, 1), 1e-9)
     est = (t1 - per) + per * max(sweeps_hint, m)
-    return {"value": n / est, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle, natural-order GS from a cold start, one step from the state in the middle of the "
-                      "timed steps cut at %d sweeps (%.1f s; set-up %.2f s, %.2f ms per sweep), scaled to the GPU's "
-                      "mean of %.0f sweeps per step: %.1f s per step" % (m, t, t1 - per, 1e3 * per, sweeps_hint, est)}
+    return {"value": n / est, "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": "FP64 oracle on %d thread(s), natural-order GS from a cold start, one step from the state in "
+                      "the middle of the timed steps cut at %d sweeps (%.1f s; set-up %.2f s, %.2f ms per sweep), "
+                      "scaled to %.0f sweeps (the GPU's cold-start sweeps on this workload): %.1f s per step"
+                      % (threads, m, t, t1 - per, 1e3 * per, sweeps_hint, est)}
+
+
+def read_traffic(path):
+    """HBM bytes per launch of the dominant kernel, from a JSON file written in the same
+    command by the rocprofv3 --pmc passes (keys: bytes_per_launch, and how it was made)."""
+    with open(path) as f:
+        return json.load(f)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: cfg1 100, the workload's step count; the 3D piles 20)")
+                    help="timed steps (default: cfg1 100, the workload's step count; the piles 20)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg1", choices=["cfg1", "cfg2", "cfg3", "cfg4"])
+    ap.add_argument("--config", default="cfg3", choices=["cfg1", "cfg2", "cfg3", "cfg4"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--settle", type=int, default=None,
-                    help="untimed settling steps of the piles (default: cfg2, cfg3 300; cfg4 200)")
+                    help="untimed settling steps of the piles under the transported cache (default: cfg2, cfg3 "
+                         "200; cfg4 200)")
     ap.add_argument("--collect", type=int, default=None,
                     help="steps whose samples train the warm start (default: cfg3 40, cfg4 10)")
-    ap.add_argument("--warm", default="none", choices=["none", "cache", "normal", "transport"],
-                    help="impulse cache warm start (ImpulsesToCache)")
+    ap.add_argument("--epochs", type=int, default=12, help="training passes over the samples")
+    ap.add_argument("--compare-steps", type=int, default=10,
+                    help="cfg3: steps of the transported-cache and learned warm starts in the comparison")
+    ap.add_argument("--cold-steps", type=int, default=2,
+                    help="cfg2/cfg3: steps of the plain coloured GS (cold start) in the comparison")
+    ap.add_argument("--warm", default=None, choices=["none", "cache", "normal", "transport"],
+                    help="impulse cache warm start of the timed steps (cfg1, cfg2; default none)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--max-iters", type=int, default=None, help="override the sweep limit (profiling)")
     ap.add_argument("--eval-every", type=int, default=None, help="override the residual period (profiling)")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of each CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=None, metavar="JSON",
+                    help="HBM bytes per launch of the dominant kernel measured by rocprofv3 --pmc in this command")
     ap.add_argument("--autotune", default="auto", choices=["auto", "on", "off"],
                     help="online autotuning of the persistent sweep launch width (scx_set_autotune); auto: on "
                          "for the piles and the ensemble, whose untimed settling steps train it, off for the "
@@ -232,9 +274,11 @@ def main():
     import scisim_amd as scx
     from scisim_amd import roofline as rl
 
+    t_start = time.perf_counter()
+
     def log(*a):
         if args.verbose:
-            print(*a, file=sys.stderr, flush=True)
+            print("[%6.1fs]" % (time.perf_counter() - t_start), *a, file=sys.stderr, flush=True)
 
     s, prm, desc = make_scene(args.config, args.seed)
     dim = s["dim"]
@@ -253,7 +297,7 @@ def main():
     it = 1
     pile = args.config in ("cfg2", "cfg3", "cfg4")
     if args.settle is None:
-        args.settle = 200 if args.config == "cfg4" else 300
+        args.settle = 200
     if args.collect is None:
         args.collect = 10 if args.config == "cfg4" else 40
     if pile and args.settle > 0:
@@ -263,28 +307,35 @@ def main():
         for k in range(args.settle):
             st = ctx.step(sp)
             if k % 25 == 24:
-                log("settle", k, "contacts", st["ncontacts"], "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
+                log("settle", k, "contacts", st["ncontacts"], "sweeps", st["sweeps"],
+                    "%.1fs" % (time.perf_counter() - t0))
         it += args.settle
         desc["settle_steps"] = args.settle
     if args.max_iters is not None:
         prm["max_iters"] = args.max_iters
     if args.eval_every is not None:
         prm["eval_every"] = args.eval_every
-    comparison = None
+    training = comparison = None
+    warm = args.warm or "none"
     if args.config in ("cfg3", "cfg4"):
-        comparison = train_learned_warm_start(ctx, s, prm, precision, args, log, compare=args.config == "cfg3")
-        args.warm = "learned"
+        training = collect_and_train(ctx, s, prm, precision, args, log)
+        warm = "learned"
+    if args.config in ("cfg2", "cfg3"):
+        comparison = compare_warm_starts(ctx, s, prm, precision, args, log) if args.config == "cfg3" else None
+        if args.config == "cfg2" and args.cold_steps > 0:
+            desc["note"] = "plain coloured GS from a cold start in every timed step"
     prm["warm_start"] = {"none": scx.WARM_NONE, "cache": scx.WARM_CACHE, "normal": scx.WARM_NORMAL,
-                         "transport": scx.WARM_TRANSPORT, "learned": scx.WARM_LEARNED}[args.warm]
-    desc["warm_start"] = args.warm
+                         "transport": scx.WARM_TRANSPORT, "learned": scx.WARM_LEARNED}[warm]
+    desc["warm_start"] = warm
     params = scx.step_params(precision=precision, iteration=it, **prm)
     for _ in range(args.warmup):
         ctx.step(params)
+    log("warmup done")
     # timed region: every step synchronises at its end; per-phase device times from HIP events
     ctx.set_timing(True)
     phase_ms = {}
     pairs_bytes = gs_bytes = bp_bytes = 0.0
-    contacts, sweeps = [], []
+    contacts, sweeps, errors = [], [], []
     fp32 = precision == scx.FP32_POLISH
     step_ms = []
     converged = 0
@@ -329,6 +380,7 @@ def main():
             gs_bytes += rl.gs_launch_bytes(dim, C, s64, s64 + 1, False)
         contacts.append(C)
         sweeps.append(st["sweeps"])
+        errors.append(st["error"])
         converged += int(st["succeeded"])
         log("step", k, st, " ".join("%s %.3f" % (name, ms) for name, ms in tm if ms > 0.02))
     wall = time.perf_counter() - t0 - sample_s
@@ -343,14 +395,9 @@ def main():
         kernel, launches, kbytes, kms = "k_pairs (grid neighbour sweep + FP64 narrow phase)", 2, pairs_bytes, pairs_ms
     else:
         kernel, launches, kbytes, kms = "k_gs (coloured Gauss-Seidel, persistent)", 1 + int(fp32), gs_bytes, gs_ms
-    # measured HBM bytes per launch of this kernel on this workload (rocprofv3 byte counters
-    # of the same command), when they exist
     traffic = None
-    try:
-        with open(os.path.join(ROOT, "scisim_amd", "measured_traffic.json")) as f:
-            traffic = json.load(f).get(desc["workload"], {}).get(kernel.split()[0], {}).get("bytes_per_launch")
-    except (OSError, ValueError):
-        traffic = None
+    if args.traffic:
+        traffic = read_traffic(args.traffic).get("bytes_per_launch")
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     # the broad phase as a whole: predictor, binning and both neighbour sweeps
     bp_ms = sum(phase_ms.get(k, 0.0) for k in ("predict", "grid", "pairs_count", "pairs_scan", "pairs_write"))
@@ -378,6 +425,15 @@ def main():
         lat = other + gs_last * ssw / max(1, sweeps[-1])
         desc["scene_sweeps_p50_p99"] = [float(np.percentile(ssw, 50)), float(np.percentile(ssw, 99))]
         desc["scene_latency_ms_p50_p99_last_step"] = [float(np.percentile(lat, 50)), float(np.percentile(lat, 99))]
+    emax = float(np.max(errors)) if errors else 0.0
+    config = dict(desc, mean_contacts=float(np.mean(contacts)), mean_sweeps=float(np.mean(sweeps)),
+                  max_iters=prm["max_iters"], steps_at_tolerance=converged,
+                  residual_max_phi_squared=emax, residual_max_phi=float(np.sqrt(emax)),
+                  phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()})
+    if training:
+        config["warm_start_training"] = training
+    if comparison:
+        config["warm_start_comparison"] = comparison
     out = {
         "metric": "body-steps/sec at FB residual<=1e-6",
         "value": value,
@@ -391,10 +447,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 sweeps + f64 residual/polish" if fp32 else "f64",
         "data": "synthetic",
-        "config": dict(desc, mean_contacts=float(np.mean(contacts)), mean_sweeps=float(np.mean(sweeps)),
-                       max_iters=prm["max_iters"], steps_at_tolerance=converged,
-                       phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()},
-                       **({"warm_start_comparison": comparison} if comparison else {})),
+        "config": config,
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": rl.HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / rl.HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": kbytes / (launches * args.steps),
@@ -403,7 +456,17 @@ def main():
     }
     ctx.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, sample, prm, args.cpu_budget, float(np.mean(sweeps)))
+        # the GPU's cold-start sweeps on this workload scale a CPU step cut short of the
+        # tolerance (cfg3: measured in the comparison; otherwise the timed steps' mean)
+        hint = float(np.mean(sweeps))
+        if comparison and "cold" in comparison:
+            hint = comparison["cold"]["mean_sweeps"]
+        cores = host_threads()
+        base = cpu_baseline(s, sample, prm, args.cpu_budget, hint, cores)
+        if cores > 1:
+            base["one_thread"] = cpu_baseline(s, sample, prm, args.cpu_budget, hint, 1)
+        out["cpu_baseline"] = base
+    log("done")
     print(json.dumps(out))
 
 

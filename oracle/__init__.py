@@ -54,6 +54,8 @@ def lib():
         L.orc_ccd_happens.argtypes = [_pd, _pd]
         L.orc_fb_err.restype = _d
         L.orc_fb_err.argtypes = [ctypes.c_int, _d, _pd, _pd]
+        L.orc_fb_phi.restype = None
+        L.orc_fb_phi.argtypes = [ctypes.c_int, _d, _pd, _pd, _pd]
         L.orc_local_solve.argtypes = [ctypes.c_int, _d, _d, _d, _pd, _pd]
         L.orc_transport.argtypes = [ctypes.c_int, _pd, _pd, _pd, _pd]
         L.orc_transport.restype = None
@@ -81,6 +83,8 @@ def lib():
         L.orc_colour.restype = ctypes.c_int
         L.orc_colour.argtypes = [ctypes.c_int, ctypes.c_int, _pi32, _pi32, _pi32, _pi32]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -135,6 +139,13 @@ def fb_err(dim, mu, r, u):
     return lib().orc_fb_err(dim, mu, _p(_f64(r)), _p(_f64(u)))
 
 
+def fb_phi(dim, mu, r, u):
+    """The Fischer-Burmeister vector phi whose squared norm fb_err returns."""
+    phi = np.zeros(dim)
+    lib().orc_fb_phi(dim, mu, _p(_f64(r)), _p(_f64(u)), _p(phi))
+    return phi
+
+
 def transport(n_old, r_old, basis):
     """Cached world impulse r_old (with its normal n_old) rotated onto the normal of
     `basis` (D x D, columns n s t) and expressed in that basis."""
@@ -169,6 +180,11 @@ def colour(nbodies, a, b):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel parts (residual evaluation, per-contact set-up)."""
+    lib().orc_set_threads(int(n))
 
 
 class Sim:

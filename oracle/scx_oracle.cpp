@@ -241,8 +241,9 @@ struct Params {
 // (So-bogus SOCLaw<D,double,true>::eval, external; semantics restated from the
 // public So-bogus source, see scisim/ConstrainedMaps/bogus/Ball2DSobogusInterface.cpp:150-168
 // for the call site). Returns ||phi||^2; the aggregated error is its max over contacts.
+// fb_phi writes the vector phi itself (D entries); fb_err returns ||phi||^2.
 template <int D>
-double fb_err(double mu, const double* r, const double* u) {
+void fb_phi(double mu, const double* r, const double* u, double* phi) {
   double uT2 = 0.0;
   for (int a = 1; a < D; ++a) uT2 += u[a] * u[a];
   const double un_hat = u[0] + mu * std::sqrt(uT2);
@@ -260,15 +261,21 @@ double fb_err(double mu, const double* r, const double* u) {
   nz1 = std::sqrt(nz1);
   const double s1 = std::sqrt(z0 + nz1);
   const double s2 = std::sqrt(std::max(0.0, z0 - nz1));
-  double phi[D];
   phi[0] = xh[0] + yh[0] - 0.5 * (s1 + s2);
   for (int a = 1; a < D; ++a) {
     const double dir = nz1 > 0.0 ? z1[a - 1] / nz1 : 0.0;
     phi[a] = xh[a] + yh[a] - 0.5 * (s1 - s2) * dir;
   }
+}
+
+template <int D>
+double fb_err(double mu, const double* r, const double* u) {
+  double phi[D];
+  fb_phi<D>(mu, r, u, phi);
   double e = 0.0;
   for (int a = 0; a < D; ++a) e += phi[a] * phi[a];
-  return e;
+  // NaN never reads as converged (std::max would drop it)
+  return std::isnan(e) ? kInf : e;
 }
 
 // Exact Coulomb local solve for a diagonal Delassus block diag(w_n, w_t, w_t):
@@ -422,6 +429,7 @@ struct Solver {
       if (D == 3)
         for (int k = 3; k < 6; ++k) minv[(size_t)i * DOF + k] = 1.0 / B.I[i];
     }
+#pragma omp parallel for schedule(static) if (C > 4096)
     for (int ci = 0; ci < C; ++ci) {
       eval_H<D>(cs[ci], H[ci]);
       // Delassus diagonal block W_ii = H_a Ma^-1 H_a^T + H_b Mb^-1 H_b^T
@@ -473,9 +481,14 @@ struct Solver {
       }
   }
 
+  // The max over contacts (an exact, order-independent reduction, so the thread count
+  // never changes the result); the reference's So-bogus interface parallelises its
+  // per-contact work with OpenMP likewise (Ball2DSobogusInterface.cpp:66-67).
   double eval(const double* r, const double* v) const {
     double e = 0.0;
-    for (size_t ci = 0; ci < cs.size(); ++ci) {
+    const int64_t C = (int64_t)cs.size();
+#pragma omp parallel for reduction(max : e) schedule(static) if (C > 4096)
+    for (int64_t ci = 0; ci < C; ++ci) {
       double u[D];
       rel_vel((int)ci, v, u);
       e = std::max(e, fb_err<D>(mus[ci], r + D * ci, u));
@@ -608,6 +621,16 @@ int orc_num_threads(void) {
 #endif
 }
 
+// Threads of the parallel parts (residual evaluation, per-contact set-up); the sweep
+// itself is sequential in the reference's natural order.
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 // Grid broad phase (reference algorithm). Returns number of pairs, or -(needed) if cap too small.
 int64_t orc_grid_pairs(int dim, uint32_t n, const double* mn, const double* mx, uint32_t* pairs, uint64_t cap) {
   std::vector<std::pair<uint32_t, uint32_t>> out;
@@ -653,6 +676,13 @@ void orc_transport(int dim, const double* n_old, const double* r_old, const doub
 
 double orc_fb_err(int dim, double mu, const double* r, const double* u) {
   return dim == 2 ? fb_err<2>(mu, r, u) : fb_err<3>(mu, r, u);
+}
+
+void orc_fb_phi(int dim, double mu, const double* r, const double* u, double* phi) {
+  if (dim == 2)
+    fb_phi<2>(mu, r, u, phi);
+  else
+    fb_phi<3>(mu, r, u, phi);
 }
 
 void orc_local_solve(int dim, double mu, double wn, double wt, const double* bt, double* r) {
@@ -853,6 +883,10 @@ typedef struct orc_ctx {
   // (old normal, world impulse) for the transported cache
   std::map<std::pair<int, int>, std::array<double, 6>> cache_bb;
   std::map<std::pair<int, int>, std::array<double, 6>> cache_pl;
+  // layout of the cached entries: 0 (alpha, beta...) or 1 (old normal, world impulse); a
+  // step that reads the other layout finds no entry (the reference's map has one mode
+  // per simulation, so this never arises there)
+  int cache_layout = 0;
   std::vector<Contact> cs;
   std::vector<int> colour;
   std::vector<double> alpha, beta, rworld;
@@ -915,7 +949,8 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
   // rotates the impulse with the normal (SymplecticEulerImpactFrictionMap.cpp:87-127).
   std::vector<double> rloc((size_t)D * C, 0.0);
   const bool caching = (P->warm_start == 1 && D == 2) || P->warm_start == 2 || P->warm_start == 3;
-  if (caching) {
+  const int layout = P->warm_start == 3 ? 1 : 0;
+  if (caching && layout == ctx->cache_layout) {
     for (int ci = 0; ci < C; ++ci) {
       const auto key = cs[ci].b >= 0 ? std::make_pair(cs[ci].a, cs[ci].b) : std::make_pair(cs[ci].plane, cs[ci].a);
       const auto& cache = cs[ci].b >= 0 ? ctx->cache_bb : ctx->cache_pl;
@@ -994,6 +1029,7 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
   }
   // cache (cacheImpulses, :222-264): alpha (NORMAL), (alpha, beta) (NORMAL_AND_FRICTION),
   // or the normal and world impulse alpha n + sum_k beta_k t_k (transported)
+  if (caching) ctx->cache_layout = layout;
   if (caching)
     for (int ci = 0; ci < C; ++ci) {
       std::array<double, 6> val{};

@@ -364,9 +364,20 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     timing_mark(c, "setup");
     HIPC(launch_colouring(c, C));
     HIPC(launch_colour_sort(c, C));
-    // second sync point: colour sizes size the persistent sweep launch
+    // second sync point: colour sizes size the persistent sweep launch; a contact graph
+    // that needs more colours than the sweeps support stops the step here, before any
+    // impulse or body state changes
     HIPC(hipMemcpyAsync(d.host_u32 + 2, d.coloff, sizeof(uint32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(d.host_u32 + kOverflowWord, &d.ctr->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    if (d.host_u32[kOverflowWord] & 8u) {
+      std::swap(d.q, d.q0);
+      std::swap(d.v, d.v0);
+      std::swap(d.w, d.w0);
+      timing_collect(c);
+      return fail(c, SCX_E_COLOURS, "contact graph needs more than %d colours (a body with too many contacts)",
+                  kMaxColours);
+    }
     timing_mark(c, "colour");
     HIPC(launch_gather_records(c, C, p->precision == SCX_PRECISION_FP32_POLISH, p->mu, c->nscenes ? d.cmu : nullptr,
                                p->warm_start != SCX_WARM_NONE ? d.crinit : nullptr));
@@ -388,6 +399,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     }
     HIPC(launch_cache_store(c, C, p->warm_start == SCX_WARM_LEARNED ? SCX_WARM_TRANSPORT : p->warm_start));
     d.prev_count = C;
+    d.prev_layout = (p->warm_start >= SCX_WARM_TRANSPORT) ? 1 : 0;
   }
   timing_mark(c, "finish");
   c->last_ncontacts = C;
@@ -396,7 +408,6 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   if (want_stats || c->timing) {
     HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
-    if (d.ctr_host->overflow & 8u) return fail(c, SCX_E_COLOURS, "contact graph needs more than %d colours", kMaxColours);
     c->last_ncolours = d.ctr_host->ncolours;
     if (st) {
       st->nentries = nentries;
@@ -491,7 +502,7 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
       break;
     }
     if (hipHostMalloc((void**)&d.ctr_host, sizeof(StepCounters)) != hipSuccess ||
-        hipHostMalloc((void**)&d.host_u32, 128 * sizeof(uint32_t)) != hipSuccess) {
+        hipHostMalloc((void**)&d.host_u32, kHostWords * sizeof(uint32_t)) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipHostMalloc failed");
       break;
     }
@@ -851,6 +862,11 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
     return fail(c, SCX_E_ARG, "scx_friction_solve: incomplete contact set");
   if (D == 3 && C > 0 && (!cs->arm0 || !cs->arm1)) return fail(c, SCX_E_ARG, "3D contacts need arms");
   if (cs->drel_extra) return fail(c, SCX_E_ARG, "tangential forcing terms (drel_extra) are not supported");
+  if (D == 3 && !inertia) return fail(c, SCX_E_ARG, "3D bodies need their inertia");
+  for (uint32_t i = 0; i < nbodies; ++i) {
+    if (!(m[i] > 0.0)) return fail(c, SCX_E_ARG, "body %u: mass must be positive", i);
+    if (D == 3 && !(inertia[i] > 0.0)) return fail(c, SCX_E_ARG, "body %u: inertia must be positive", i);
+  }
   for (uint32_t k = 0; k < C; ++k) {
     if (cs->body0[k] < 0 || (uint32_t)cs->body0[k] >= nbodies || cs->body1[k] < -1 ||
         cs->body1[k] >= (int32_t)nbodies || cs->body0[k] == cs->body1[k])
@@ -928,7 +944,11 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
     HIPC(launch_colouring(c, C));
     HIPC(launch_colour_sort(c, C));
     HIPC(hipMemcpyAsync(d.host_u32 + 2, d.coloff, sizeof(uint32_t) * (kMaxColours + 1), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(d.host_u32 + kOverflowWord, &d.ctr->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    if (d.host_u32[kOverflowWord] & 8u)
+      return fail(c, SCX_E_COLOURS, "contact graph needs more than %d colours (a body with too many contacts)",
+                  kMaxColours);
     HIPC(launch_gather_records(c, C, precision == SCX_PRECISION_FP32_POLISH, 0.0, d.calpha, d.crinit));
   }
   scx_step_params p{};
@@ -955,7 +975,6 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   }
   HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
-  if (C > 0 && (d.ctr_host->overflow & 8u)) return fail(c, SCX_E_COLOURS, "contact graph needs more than 64 colours");
   for (uint32_t i = 0; i < nbodies; ++i) {
     if (v2)
       for (int q = 0; q < D; ++q) v2[(size_t)D * i + q] = bv[rec * i + q];

@@ -475,15 +475,25 @@ __global__ void __launch_bounds__(kBlock) k_gather(uint32_t C, int dim, const ui
 
 // Cache keys in active-set order, ascending: body pairs (i << 32 | j), then plane
 // contacts (1 << 63 | plane << 32 | i) -- the (i, j) and (plane, i) keys of
-// ball2d/ConstraintCache.cpp:20-123.
+// ball2d/ConstraintCache.cpp:20-123. In an ensemble the plane contacts are ordered by
+// the scene-local plane index (all scenes' plane-0 contacts, then plane 1, ...), so the
+// key takes that local index: with global body ids the keys stay unique and ascend in
+// that order, which the binary search of k_warm_lookup needs.
 __global__ void __launch_bounds__(kBlock) k_contact_keys(uint32_t C, const uint32_t* __restrict__ ca,
                                                          const uint32_t* __restrict__ cb,
                                                          const int32_t* __restrict__ cplane,
+                                                         const uint32_t* __restrict__ bscene,
+                                                         const uint32_t* __restrict__ scene_poff,
                                                          unsigned long long* __restrict__ key) {
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= C) return;
-  key[k] = (cb[k] != kNoBody) ? (((unsigned long long)ca[k] << 32) | cb[k])
-                              : ((1ull << 63) | ((unsigned long long)(uint32_t)cplane[k] << 32) | ca[k]);
+  if (cb[k] != kNoBody) {
+    key[k] = ((unsigned long long)ca[k] << 32) | cb[k];
+  } else {
+    uint32_t plane = (uint32_t)cplane[k];
+    if (bscene) plane -= scene_poff[bscene[ca[k]]];
+    key[k] = (1ull << 63) | ((unsigned long long)plane << 32) | ca[k];
+  }
 }
 
 // Cached world impulse r_old rotated by the minimal rotation taking the old normal n_old
@@ -780,9 +790,15 @@ hipError_t launch_sample_targets(scx_ctx* c, uint32_t C) {
 hipError_t launch_warm_start(scx_ctx* c, uint32_t C, int mode) {
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
-  k_contact_keys<<<nblk, kBlock, 0, c->stream>>>(C, c->d.ca, c->d.cb, c->d.cplane, c->d.cur_key);
-  k_warm_lookup<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mode, c->d.cur_key, c->d.prev_key, c->d.prev_count,
-                                                c->d.prev_r, c->d.basis, c->d.crinit);
+  k_contact_keys<<<nblk, kBlock, 0, c->stream>>>(C, c->d.ca, c->d.cb, c->d.cplane,
+                                                 c->nscenes ? c->d.bscene : nullptr, c->d.scene_poff,
+                                                 c->d.cur_key);
+  // entries written in the other layout are not read (a context that switches modes
+  // starts that step cold)
+  const int layout = (mode >= SCX_WARM_TRANSPORT) ? 1 : 0;
+  const uint32_t P = (layout == c->d.prev_layout) ? c->d.prev_count : 0u;
+  k_warm_lookup<<<nblk, kBlock, 0, c->stream>>>(C, c->dim, mode, c->d.cur_key, c->d.prev_key, P, c->d.prev_r,
+                                                c->d.basis, c->d.crinit);
   return hipGetLastError();
 }
 

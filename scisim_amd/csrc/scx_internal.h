@@ -32,6 +32,8 @@ constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHid;
 constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHid * kMlpOut;
 constexpr int kMlpParams = kMlpOffB3 + kMlpOut;
 constexpr int kMlpSseBlocks = 1024;   // most workgroups of a training pass (loss partials)
+constexpr int kHostWords = 128;       // pinned host scratch words (DeviceBuffers::host_u32)
+constexpr int kOverflowWord = 80;     // host_u32 word that receives StepCounters::overflow
 
 // Grid parameters, computed on the device each step (reference formula of
 // ball2d/SpatialGridDetector.cpp:76-104: min corner - 2e-6, h = max axis of the summed
@@ -188,6 +190,9 @@ struct DeviceBuffers {
   unsigned long long* prev_key = nullptr;  // cap_cache
   double* prev_r = nullptr;    // cached (alpha, beta...) per key, dim each (cap_cache)
   uint32_t prev_count = 0;
+  int prev_layout = 0;         // layout of the cached entries: 0 (alpha, beta...) for CACHE /
+                               // NORMAL, 1 (old normal, world impulse) for TRANSPORT / LEARNED;
+                               // a lookup in the other layout finds no entry
   uint32_t cap_cache = 0;
   unsigned long long* cur_key = nullptr;   // this step's keys (cap_contacts)
   // learned warm start
@@ -214,7 +219,9 @@ struct DeviceBuffers {
   StepCounters* ctr = nullptr;
   uint32_t* barrier = nullptr;       // grid barrier state of the persistent kernels (zeroed once)
   StepCounters* ctr_host = nullptr;  // pinned
-  uint32_t* host_u32 = nullptr;      // pinned scratch
+  uint32_t* host_u32 = nullptr;      // pinned scratch (kHostWords): [0..1] active-set sizes,
+                                     // [2..66] colour offsets, [kOverflowWord] overflow bits,
+                                     // [100..101] sweep counters of the autotuner
 };
 
 }  // namespace scx

@@ -165,7 +165,9 @@ __device__ __forceinline__ T fb_err(T mu, const T* r, const T* u) {
     const T p = xh[a] + yh[a] - (T)0.5 * (s1 - s2) * dir;
     e += p * p;
   }
-  return e;
+  // a NaN residual (e.g. from non-finite velocities) counts as infinitely far from the
+  // solution: the max reductions drop NaN operands, so it must never read as converged
+  return __builtin_isnan(e) ? (T)INFINITY : e;
 }
 
 // Body records per slot: 2D one vector [v (3) | minv], 3D two adjacent vectors
