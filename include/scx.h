@@ -46,6 +46,18 @@ extern "C" {
 #define SCX_INTEGRATOR_SYMPLECTIC_EULER 0
 #define SCX_INTEGRATOR_VERLET 1
 
+/* Staggering of the impact-friction map: the "staggering" attribute of the reference's
+ * <sobogus_friction_solver> (rigidbody3dutils/RigidBody3DSceneParser.cpp:1865-1962). */
+#define SCX_STAGGER_GEOMETRIC 0        /* GeometricImpactFrictionMap::flow (GeometricImpactFrictionMap.cpp:267-442):
+                                          solve on v0, then the unconstrained map from q0 with v2 */
+#define SCX_STAGGER_SYMPLECTIC_EULER 1 /* SymplecticEulerImpactFrictionMap::flow
+                                          (SymplecticEulerImpactFrictionMap.cpp:201-407): the force
+                                          increment dt M^-1 F(q0) enters the solve, q1 = q0 + dt v1;
+                                          needs CoR = 0 (RigidBody3DSceneParser.cpp:1881-1888) */
+#define SCX_STAGGER_STABILIZED 2       /* StabilizedImpactFrictionMap::flow (StabilizedImpactFrictionMap.cpp:48-183):
+                                          cold solve on the predictor velocity, q1 = q0 + dt v1; no
+                                          impulse cache (warm_start must be SCX_WARM_NONE) */
+
 #define SCX_PRECISION_FP64 0       /* every sweep in FP64 */
 #define SCX_PRECISION_FP32_POLISH 1 /* FP32 sweeps, then FP64 polishing sweeps to tolerance */
 
@@ -92,6 +104,10 @@ typedef struct scx_step_params {
   int32_t precision;    /* SCX_PRECISION_* */
   int32_t warm_start;   /* SCX_WARM_* */
   uint32_t iteration;   /* step index (informational) */
+  int32_t staggering;   /* SCX_STAGGER_* (0 = geometric) */
+  int32_t stabilize;    /* symplectic Euler: add min(0, gap(q0)) / dt to each normal forcing term
+                           (SymplecticEulerImpactFrictionMap.cpp:309-327) */
+  double penetration_threshold; /* symplectic Euler with stabilize: gaps of smaller magnitude count as 0 */
 } scx_step_params;
 
 typedef struct scx_step_stats {
@@ -148,7 +164,8 @@ int scx_set_scenes(scx_ctx* ctx, uint32_t nscenes, const uint32_t* body_off, con
 int scx_get_scene_stats(scx_ctx* ctx, uint32_t nscenes, uint32_t* ncontacts, double* error, uint32_t* sweeps);
 int scx_get_state(scx_ctx* ctx, double* q, double* v, double* omega);
 
-/* One GeometricImpactFrictionMap step on the resident state. stats may be NULL. */
+/* One impact-friction map step on the resident state (GeometricImpactFrictionMap, or the
+ * staggering named by p->staggering). stats may be NULL. */
 int scx_step(scx_ctx* ctx, const scx_step_params* p, scx_step_stats* stats);
 /* Several steps without host synchronisation between them except where the
  * pipeline needs counts; stats (may be NULL) receives the last step's stats. */
@@ -160,7 +177,8 @@ int scx_run(scx_ctx* ctx, const scx_step_params* p, uint32_t nsteps, scx_step_st
  * ((dim-1) per contact), colour. Any output pointer may be NULL. */
 int scx_get_contacts(scx_ctx* ctx, uint32_t cap, int32_t* body0, int32_t* body1, int32_t* plane, double* bases,
                      double* alpha, double* beta, int32_t* colour, uint32_t* ncontacts);
-/* Last step's post-impact velocity v2 (dim*n; 3D: plus omega2 3n if not NULL). */
+/* Last step's post-impact velocity v2 (dim*n; 3D: plus omega2 3n if not NULL). Under the
+ * symplectic Euler and stabilized staggerings this is the step's final velocity v1. */
 int scx_get_v2(scx_ctx* ctx, double* v2, double* omega2);
 
 /* An explicit active set for scx_friction_solve, in the reference's conventions

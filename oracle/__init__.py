@@ -25,7 +25,8 @@ class OrcParams(ctypes.Structure):
     _fields_ = [("dt", _d), ("g", _d * 3), ("mu", _d), ("CoR", _d), ("tol", _d),
                 ("max_iters", ctypes.c_uint32), ("eval_every", ctypes.c_uint32),
                 ("integrator", ctypes.c_int32), ("order", ctypes.c_int32),
-                ("warm_start", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("warm_start", ctypes.c_int32), ("staggering", ctypes.c_int32), ("stabilize", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("penetration_threshold", _d)]
 
 
 class OrcStats(ctypes.Structure):
@@ -211,10 +212,13 @@ class Sim:
             self._ctx = None
 
     def step(self, dt, g, mu, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1, integrator=0, order=0,
-             warm_start=0):
+             warm_start=0, staggering=0, stabilize=0, penetration_threshold=0.0):
+        """One map step: staggering 0 geometric, 1 symplectic Euler (stabilize adds the
+        min(0, gap)/dt term), 2 stabilized."""
         P = OrcParams(dt=dt, g=(_d * 3)(*(list(g) + [0.0] * (3 - len(g)))), mu=mu, CoR=CoR, tol=tol,
                       max_iters=max_iters, eval_every=eval_every, integrator=integrator, order=order,
-                      warm_start=warm_start)
+                      warm_start=warm_start, staggering=staggering, stabilize=stabilize,
+                      penetration_threshold=penetration_threshold)
         S = OrcStats()
         rc = lib().orc_step(self._ctx, self.n, _p(self.q), _p(self.v), _p(self.w), _p(self.r), _p(self.m),
                             _p(self.inertia), self.nplanes, _p(self.px), _p(self.pn), ctypes.byref(P),
@@ -225,10 +229,11 @@ class Sim:
                 "sweeps": S.sweeps, "error": S.error, "succeeded": bool(S.succeeded)}
 
     def run(self, nsteps, dt, g, mu, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1, integrator=0, order=0,
-            warm_start=0):
+            warm_start=0, staggering=0, stabilize=0, penetration_threshold=0.0):
         P = OrcParams(dt=dt, g=(_d * 3)(*(list(g) + [0.0] * (3 - len(g)))), mu=mu, CoR=CoR, tol=tol,
                       max_iters=max_iters, eval_every=eval_every, integrator=integrator, order=order,
-                      warm_start=warm_start)
+                      warm_start=warm_start, staggering=staggering, stabilize=stabilize,
+                      penetration_threshold=penetration_threshold)
         S = OrcStats()
         ms = ctypes.c_uint32(0)
         nf = ctypes.c_uint32(0)

@@ -221,6 +221,8 @@ struct Contact {
   double pa[3];      // arm of body a (3D): contact point - x_a, at q0
   double pb[3];      // arm of body b (3D)
   double ndotv;      // evalNdotV at (q0, v0)
+  double nrel;       // extra normal forcing (nrel_extra of FrictionSolver::solve)
+  double drel[2];    // extra tangential forcing (drel_extra), D-1 used
 };
 
 struct Params {
@@ -445,10 +447,14 @@ struct Solver {
       wn[ci] = W[0][0];
       wt[ci] = (D == 2) ? W[1][1] : 0.5 * (W[1][1] + W[2][2]);
       // Constraint::computeForcingTerm (scisim/Constraints/Constraint.cpp:18-34) with
-      // static geometry (nrel = drel = 0), in the local frame: (CoR * n.v0, 0...).
-      forcing[ci] = CoR * cs[ci].ndotv;
+      // static geometry (kinematic nrel = drel = 0) plus the caller's extra terms
+      // (Sobogus.cpp:893-905), in the local frame: (CoR * n.v + nrel, drel...).
+      forcing[ci] = CoR * cs[ci].ndotv + cs[ci].nrel;
     }
+    for (int ci = 0; ci < C; ++ci)
+      for (int k = 0; k < D - 1; ++k) has_drel = has_drel || cs[ci].drel[k] != 0.0;
   }
+  bool has_drel = false;
 
   // u = H_a v_a - H_b v_b + w, in the local frame.
   void rel_vel(int ci, const double* v, double* u) const {
@@ -464,6 +470,8 @@ struct Solver {
       u[row] = s;
     }
     u[0] += forcing[ci];
+    if (has_drel)
+      for (int k = 1; k < D; ++k) u[k] += c.drel[k - 1];
   }
 
   void apply(int ci, const double* dr, double* v) const {
@@ -599,7 +607,10 @@ typedef struct orc_params {
   int32_t integrator;
   int32_t order;
   int32_t warm_start;
+  int32_t staggering;   // 0 geometric, 1 symplectic Euler, 2 stabilized
+  int32_t stabilize;    // symplectic Euler: min(0, gap(q0)) / dt added to the normal forcing
   int32_t pad;
+  double penetration_threshold;
 } orc_params;
 
 typedef struct orc_stats {
@@ -901,28 +912,81 @@ orc_ctx* orc_create(int dim) {
 }
 void orc_destroy(orc_ctx* c) { delete c; }
 
-// One step of GeometricImpactFrictionMap::flow
-// (scisim/ConstrainedMaps/GeometricImpactFrictionMap.cpp:267-442) for circles/spheres.
-// q, v (D*n) and w (3D: 3n angular velocity) are updated in place.
+// Gap function at q (evaluateGapFunction): ball2d/Constraints/BallBallConstraint.cpp:76-86,
+// BallStaticPlaneConstraint.cpp:58-63, rigidbody3d/Constraints/SphereSphereConstraint.cpp:28-31,
+// StaticPlaneSphereConstraint.cpp:31-35.
+static double gap(int D, const Contact& c, const double* q, const double* r, const double* px, const double* pn) {
+  if (c.b >= 0) {
+    double d2 = 0.0;
+    for (int a = 0; a < D; ++a) {
+      const double d = q[(size_t)D * c.a + a] - q[(size_t)D * c.b + a];
+      d2 = (a == 0) ? d * d : d2 + d * d;
+    }
+    // 2D subtracts the radius sum, 3D each radius in turn
+    return D == 2 ? std::sqrt(d2) - (r[c.a] + r[c.b]) : (std::sqrt(d2) - r[c.a]) - r[c.b];
+  }
+  double nn[3] = {pn[(size_t)D * c.plane], pn[(size_t)D * c.plane + 1], D == 3 ? pn[(size_t)D * c.plane + 2] : 0.0};
+  if (D == 2) normalize<2>(nn); else normalize<3>(nn);
+  double s = 0.0;
+  for (int a = 0; a < D; ++a) {
+    const double d = q[(size_t)D * c.a + a] - px[(size_t)D * c.plane + a];
+    s = (a == 0) ? nn[0] * d : s + nn[a] * d;
+  }
+  return s - r[c.a];
+}
+
+// One step of an impact-friction map for circles/spheres, q, v (D*n) and w (3D: 3n
+// angular velocity) updated in place:
+//   staggering 0: GeometricImpactFrictionMap::flow
+//     (scisim/ConstrainedMaps/GeometricImpactFrictionMap.cpp:267-442);
+//   staggering 1: SymplecticEulerImpactFrictionMap::flow
+//     (scisim/ConstrainedMaps/SymplecticEulerImpactFrictionMap.cpp:201-407): force increment
+//     vdelta = dt M^-1 F(q0), predictor q0 + dt (v0 + vdelta), bases at (q0, v0), forcing
+//     nrel = N^T vdelta (+ min(0, g(q0)) / dt with stabilize), drel = D^T vdelta, solve on v0,
+//     v1 = v2 + vdelta, q1 = q0 + dt v1;
+//   staggering 2: StabilizedImpactFrictionMap::flow
+//     (scisim/ConstrainedMaps/StabilizedImpactFrictionMap.cpp:48-183): predictor (q1, v1)
+//     = U(q0, v0), bases at (q0, v1), cold solve on v1 (no cache), v1 = v2, q1 = q0 + dt v1.
 int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double* r, const double* m,
              const double* I, int nplanes, const double* px, const double* pn, const orc_params* P,
              orc_stats* st) {
   const int D = ctx->dim;
+  const int stag = P->staggering;
   std::memset(st, 0, sizeof(*st));
-  // "Caching in 3 space not yet supported." (GeometricImpactFrictionMap.cpp:258-262)
+  // "Caching in 3 space not yet supported." (GeometricImpactFrictionMap.cpp:258-262); the
+  // 3D cache of the symplectic Euler map is the transported one (warm_start 3)
   if (P->warm_start == 1 && D == 3) return -3;
+  if (stag < 0 || stag > 2) return -4;
+  // rigidbody3dutils/RigidBody3DSceneParser.cpp:1881-1888: the symplectic Euler staggering
+  // needs CoR = 0; the stabilized map has no impulse cache (StabilizedImpactFrictionMap.cpp:107-109)
+  if (stag == 1 && P->CoR != 0.0) return -5;
+  if (stag == 2 && P->warm_start != 0) return -6;
   std::vector<double> q0(q, q + (size_t)D * n), v0(v, v + (size_t)D * n);
   std::vector<double> w0;
   if (D == 3) w0.assign(w, w + (size_t)3 * n);
-  std::vector<double> q1((size_t)D * n), v1((size_t)D * n);
-  // predictor (:279)
-  if (D == 2)
+  std::vector<double> q1((size_t)D * n), v1((size_t)D * n), vd;
+  if (stag == 1) {
+    // vdelta = dt * (M^-1 F), F = m g (SymplecticEulerImpactFrictionMap.cpp:222-229); the
+    // predictor is linearInertialConfigurationUpdate(q0, v0 + vdelta) (:232)
+    vd.resize((size_t)D * n);
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < D; ++a) {
+        const size_t k = (size_t)D * i + a;
+        vd[k] = P->dt * ((1.0 / m[i]) * (m[i] * P->g[a]));
+        v1[k] = v0[k] + vd[k];
+        q1[k] = q0[k] + P->dt * v1[k];
+      }
+  } else if (D == 2) {
+    // predictor (GeometricImpactFrictionMap.cpp:279, StabilizedImpactFrictionMap.cpp:60)
     umap_flow<2>(P->integrator, n, m, P->g, P->dt, q0.data(), v0.data(), q1.data(), v1.data());
-  else
+  } else {
     umap_flow<3>(P->integrator, n, m, P->g, P->dt, q0.data(), v0.data(), q1.data(), v1.data());
-  // active set (:283) and bases at (q0, v0) (:304)
+  }
+  // active set at (q0, q1) and bases at (q0, v0) (geometric :283,:304; symplectic Euler
+  // :235,:279), or at (q0, v1) (stabilized :63,:85)
+  const std::vector<double>& vb = (stag == 2) ? v1 : v0;
   std::vector<Contact>& cs = ctx->cs;
-  active_set(D, n, q0.data(), q1.data(), v0.data(), D == 3 ? w0.data() : nullptr, r, nplanes, px, pn, cs,
+  active_set(D, n, q0.data(), q1.data(), vb.data(), D == 3 ? w0.data() : nullptr, r, nplanes, px, pn, cs,
              &st->npairs);
   const int C = (int)cs.size();
   st->ncontacts = C;
@@ -932,14 +996,42 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
   ctx->colour.assign(C, 0);
   ctx->hist.clear();
   if (C == 0) {
-    // empty active set: keep the predictor, clear the cache (:286-298)
-    ctx->cache_bb.clear();
-    ctx->cache_pl.clear();
+    // empty active set: keep the predictor; the geometric and symplectic Euler maps clear
+    // their cache (:286-298; SymplecticEulerImpactFrictionMap.cpp:239-253), the stabilized
+    // map has none (:66-78)
+    if (stag != 2) {
+      ctx->cache_bb.clear();
+      ctx->cache_pl.clear();
+    }
     std::copy(q1.begin(), q1.end(), q);
     std::copy(v1.begin(), v1.end(), v);
-    ctx->v2 = v0;
+    ctx->v2 = (stag == 0) ? v0 : v1;
     st->succeeded = 1;
     return 0;
+  }
+  if (stag == 1) {
+    // nrel = N^T vdelta, drel = D^T vdelta (:296-307): sparse column dot products over the
+    // rows of body a then body b (b > a), +n / -n, +t / -t; the angular rows of vdelta are
+    // zero (no torque)
+    for (Contact& c : cs) {
+      for (int row = 0; row < D; ++row) {
+        const double* e = c.basis + (size_t)D * row;  // n, then the tangents
+        double acc = 0.0;
+        for (int a = 0; a < D; ++a) acc = acc + e[a] * vd[(size_t)D * c.a + a];
+        if (c.b >= 0)
+          for (int a = 0; a < D; ++a) acc = acc + (-e[a]) * vd[(size_t)D * c.b + a];
+        if (row == 0)
+          c.nrel = acc;
+        else
+          c.drel[row - 1] = acc;
+      }
+      if (P->stabilize) {
+        // g0 = min(0, g(q0)), zero below the penetration threshold, nrel += g0 / dt (:309-326)
+        double g0 = std::min(0.0, gap(D, c, q0.data(), r, px, pn));
+        if (std::fabs(g0) < P->penetration_threshold) g0 = 0.0;
+        c.nrel = c.nrel + g0 / P->dt;
+      }
+    }
   }
   std::vector<double> mu(C, P->mu);
   // warm start (initializeImpulses, :174-220): the cached (alpha, beta) of the same
@@ -993,8 +1085,9 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
   // velocities in solver layout [lin | ang] per body
   const int DOF = (D == 2) ? 2 : 6;
   std::vector<double> vs((size_t)DOF * n), vout((size_t)DOF * n);
+  // the solve starts from v0, or from the predictor velocity in the stabilized map (:117)
   for (int i = 0; i < n; ++i) {
-    for (int a = 0; a < D; ++a) vs[(size_t)DOF * i + a] = v0[(size_t)D * i + a];
+    for (int a = 0; a < D; ++a) vs[(size_t)DOF * i + a] = vb[(size_t)D * i + a];
     if (D == 3)
       for (int a = 0; a < 3; ++a) vs[(size_t)DOF * i + 3 + a] = w0[(size_t)3 * i + a];
   }
@@ -1057,11 +1150,23 @@ int orc_step(orc_ctx* ctx, int n, double* q, double* v, double* w, const double*
     if (D == 3)
       for (int a = 0; a < 3; ++a) w[(size_t)3 * i + a] = vout[(size_t)DOF * i + 3 + a];
   }
-  ctx->v2 = v2;
-  if (D == 2)
-    umap_flow<2>(P->integrator, n, m, P->g, P->dt, q0.data(), v2.data(), q, v);
-  else
-    umap_flow<3>(P->integrator, n, m, P->g, P->dt, q0.data(), v2.data(), q, v);
+  if (stag == 0) {
+    ctx->v2 = v2;
+    if (D == 2)
+      umap_flow<2>(P->integrator, n, m, P->g, P->dt, q0.data(), v2.data(), q, v);
+    else
+      umap_flow<3>(P->integrator, n, m, P->g, P->dt, q0.data(), v2.data(), q, v);
+    return 0;
+  }
+  // symplectic Euler: v1 = v2 + vdelta (:368); both: q1 = q0 + dt v1
+  // (linearInertialConfigurationUpdate, ball2d/Ball2DSim.cpp:92-98,
+  // rigidbody3d/UnconstrainedMaps/IntegrationTools.cpp:36-51)
+  for (size_t k = 0; k < (size_t)D * n; ++k) {
+    const double vk = (stag == 1) ? v2[k] + vd[k] : v2[k];
+    v[k] = vk;
+    q[k] = q0[k] + P->dt * vk;
+  }
+  ctx->v2.assign(v, v + (size_t)D * n);
   return 0;
 }
 

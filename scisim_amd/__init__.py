@@ -29,6 +29,9 @@ WARM_NONE = 0
 WARM_CACHE = 1
 WARM_NORMAL = 2
 WARM_TRANSPORT = 3
+STAGGER_GEOMETRIC = 0
+STAGGER_SYMPLECTIC_EULER = 1
+STAGGER_STABILIZED = 2
 
 # Every symbol include/scx.h declares.
 EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_potential_overlaps", "scx_set_bodies",
@@ -56,7 +59,8 @@ class ScxConfig(ctypes.Structure):
 class ScxStepParams(ctypes.Structure):
     _fields_ = [("dt", _d), ("g", _d * 3), ("mu", _d), ("CoR", _d), ("tol", _d), ("max_iters", ctypes.c_uint32),
                 ("eval_every", ctypes.c_uint32), ("integrator", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("warm_start", ctypes.c_int32), ("iteration", ctypes.c_uint32)]
+                ("warm_start", ctypes.c_int32), ("iteration", ctypes.c_uint32), ("staggering", ctypes.c_int32),
+                ("stabilize", ctypes.c_int32), ("penetration_threshold", _d)]
 
 
 class ScxStepStats(ctypes.Structure):
@@ -131,11 +135,13 @@ def _f64(a):
 
 
 def step_params(dt, g, mu, CoR=0.0, tol=1e-6, max_iters=5000, eval_every=1, integrator=SYMPLECTIC_EULER,
-                precision=FP64, warm_start=WARM_NONE, iteration=1):
+                precision=FP64, warm_start=WARM_NONE, iteration=1, staggering=None, stabilize=0,
+                penetration_threshold=0.0):
     gg = list(g) + [0.0] * (3 - len(g))
     return ScxStepParams(dt=dt, g=(_d * 3)(*gg), mu=mu, CoR=CoR, tol=tol, max_iters=max_iters,
                          eval_every=eval_every, integrator=integrator, precision=precision, warm_start=warm_start,
-                         iteration=iteration)
+                         iteration=iteration, staggering=STAGGER_GEOMETRIC if staggering is None else staggering,
+                         stabilize=stabilize, penetration_threshold=penetration_threshold)
 
 
 class Context:

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "scx_device.h"
+#include "scx_flow.h"
 #include "scx_internal.h"
 
 using namespace scx;
@@ -85,6 +86,7 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
   HIPC(dalloc(&d.v0, (size_t)D * N));
   HIPC(dalloc(&d.w0, 3 * (size_t)N));
   HIPC(dalloc(&d.q1p, (size_t)D * N));
+  HIPC(dalloc(&d.v1p, (size_t)D * N));
   HIPC(dalloc(&d.v2, (size_t)D * N));
   HIPC(dalloc(&d.w2, 3 * (size_t)N));
   HIPC(dalloc(&d.bv64, 8 * (size_t)N));
@@ -198,7 +200,7 @@ int alloc_scan_tmp(scx_ctx* c) {
 
 void free_all(scx_ctx* c) {
   DeviceBuffers& d = c->d;
-  double* dbl[] = {d.q, d.v, d.w, d.r, d.m, d.inertia, d.q0, d.v0, d.w0, d.q1p, d.v2, d.w2, d.bv64,
+  double* dbl[] = {d.q, d.v, d.w, d.r, d.m, d.inertia, d.q0, d.v0, d.w0, d.q1p, d.v1p, d.v2, d.w2, d.bv64,
                    d.px, d.pn, d.aabb_min, d.aabb_max, d.red, d.basis, d.cla, d.clb, d.cwn, d.cwt, d.cforcing,
                    d.calpha, d.cbeta, d.crinit, d.rbest, d.partial, d.prev_r, d.scene_mu, d.scene_shift,
                    d.scene_err, d.cmu};
@@ -283,8 +285,19 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   std::swap(d.q, d.q0);
   std::swap(d.v, d.v0);
   std::swap(d.w, d.w0);
-  // predictor + boxes (GeometricImpactFrictionMap.cpp:279)
-  HIPC(launch_predict_aabb(c, d.q0, d.v0, d.m, p->dt, p->g, p->integrator, d.q1p));
+  // predictor + boxes: the unconstrained map (GeometricImpactFrictionMap.cpp:279,
+  // StabilizedImpactFrictionMap.cpp:60) or q0 + dt (v0 + dt M^-1 F)
+  // (SymplecticEulerImpactFrictionMap.cpp:222-232). The symplectic Euler map's solve on v0
+  // with forcing N^T vdelta, D^T vdelta (:296-308) is the same problem as a solve that starts
+  // from v0 + vdelta with no extra forcing, whose result is already v1 = v2 + vdelta (:368):
+  // the force increment is folded into the start velocity, so the sweep records carry no
+  // tangential forcing. The stabilized map solves on the predictor velocity (:117).
+  const int stag = p->staggering;
+  HIPC(launch_predict_aabb(c, d.q0, d.v0, d.m, p->dt, p->g,
+                           stag == SCX_STAGGER_SYMPLECTIC_EULER ? kFlowForceIncrement : p->integrator, d.q1p,
+                           stag == SCX_STAGGER_GEOMETRIC ? nullptr : d.v1p));
+  c->vbasis = (stag == SCX_STAGGER_STABILIZED) ? d.v1p : d.v0;
+  c->vsolve = (stag == SCX_STAGGER_GEOMETRIC) ? d.v0 : d.v1p;
   timing_mark(c, "predict");
   // broad phase binning
   HIPC(launch_grid_build(c, n, false));
@@ -340,7 +353,8 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   }
   const uint32_t C = (uint32_t)C64;
   if (C > 0) {
-    HIPC(launch_contact_setup(c, nbb, npl, p->CoR, p->mu));
+    const bool stab = stag == SCX_STAGGER_SYMPLECTIC_EULER && p->stabilize;
+    HIPC(launch_contact_setup(c, nbb, npl, p->CoR, stab ? p->dt : 0.0, p->penetration_threshold));
     HIPC(launch_contact_mu(c, C));
     // warm start from the impulse cache (GeometricImpactFrictionMap.cpp:306); the learned
     // warm start reads the transported cache as an input and predicts the impulses
@@ -386,10 +400,15 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
   HIPC(launch_solve(c, C, d.host_u32 + 2, p));
   timing_mark(c, "gs_fp64");
   HIPC(launch_finish(c, C, p));
+  // the stabilized map with an empty active set keeps the unconstrained predictor
+  // (StabilizedImpactFrictionMap.cpp:66-78); its velocity is already v1p
+  if (stag == SCX_STAGGER_STABILIZED && C == 0)
+    HIPC(hipMemcpyAsync(d.q, d.q1p, sizeof(double) * D * n, hipMemcpyDeviceToDevice, s));
   // cache this step's impulses (GeometricImpactFrictionMap.cpp:419); an empty active set
   // empties the cache (:286-288); otherwise a step without caching (NONE) leaves it
-  // untouched, as the reference's map never touches its cache in that mode
-  if (p->warm_start != SCX_WARM_NONE || C == 0) d.prev_count = 0;
+  // untouched, as the reference's map never touches its cache in that mode; the
+  // stabilized map has no cache
+  if (stag != SCX_STAGGER_STABILIZED && (p->warm_start != SCX_WARM_NONE || C == 0)) d.prev_count = 0;
   if (p->warm_start != SCX_WARM_NONE && C > 0) {
     if (C > d.cap_cache) {
       const uint32_t want = C + C / 4 + 1024;
@@ -439,6 +458,14 @@ int check_params(scx_ctx* c, const scx_step_params* p) {
     return fail(c, SCX_E_ARG, "unknown precision mode %d", p->precision);
   if (p->warm_start < SCX_WARM_NONE || p->warm_start > SCX_WARM_LEARNED)
     return fail(c, SCX_E_ARG, "unknown warm start mode %d", p->warm_start);
+  if (p->staggering < SCX_STAGGER_GEOMETRIC || p->staggering > SCX_STAGGER_STABILIZED)
+    return fail(c, SCX_E_ARG, "unknown staggering %d", p->staggering);
+  // RigidBody3DSceneParser.cpp:1881-1888
+  if (p->staggering == SCX_STAGGER_SYMPLECTIC_EULER && p->CoR != 0.0)
+    return fail(c, SCX_E_ARG, "the symplectic Euler staggering needs CoR = 0");
+  if (p->staggering == SCX_STAGGER_STABILIZED && p->warm_start != SCX_WARM_NONE)
+    return fail(c, SCX_E_ARG, "the stabilized staggering has no impulse cache (warm_start must be SCX_WARM_NONE)");
+  if (!(p->penetration_threshold >= 0.0)) return fail(c, SCX_E_ARG, "penetration_threshold must be non-negative");
   if (p->warm_start == SCX_WARM_LEARNED && !c->d.mlp_ready)
     return fail(c, SCX_E_STATE, "learned warm start needs a model (scx_set_warm_model or scx_train_warm_model)");
   // GeometricImpactFrictionMap.cpp:258-262: "Caching in 3 space not yet supported."
@@ -930,6 +957,7 @@ int scx_friction_solve(scx_ctx* c, uint32_t nbodies, const double* m, const doub
   HIPC(hipMemcpyAsync(d.v0, v0, sizeof(double) * D * nbodies, hipMemcpyHostToDevice, s));
   HIPC(hipMemcpyAsync(d.w0, w0.data(), sizeof(double) * 3 * nbodies, hipMemcpyHostToDevice, s));
   HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+  c->vbasis = c->vsolve = d.v0;
   if (C > 0) {
     HIPC(hipMemcpyAsync(d.ca, ca.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
     HIPC(hipMemcpyAsync(d.cb, cb.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));

@@ -123,7 +123,8 @@ __global__ void __launch_bounds__(kBlock) k_predict_aabb(uint32_t n, const uint3
                                                          const double* __restrict__ q0, const double* __restrict__ v0,
                                                          const double* __restrict__ m, const double* __restrict__ r,
                                                          double dt, double gx, double gy, double gz, int integrator,
-                                                         double* __restrict__ q1p, float4* __restrict__ tbox,
+                                                         double* __restrict__ q1p, double* __restrict__ v1p,
+                                                         float4* __restrict__ tbox,
                                                          double4* __restrict__ tnar, double* __restrict__ tr,
                                                          const uint32_t* __restrict__ bscene,
                                                          const double* __restrict__ shift,
@@ -135,7 +136,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_aabb(uint32_t n, const uint3
   if (valid) {
     const uint32_t i = order[t];
     const double mi = m[i], ri = r[i];
-    double qa[D], va[D], q1[D];
+    double qa[D], va[D], q1[D], v1[D];
     if (D == 2) {
       const double2 qq = reinterpret_cast<const double2*>(q0)[i];
       const double2 vv = reinterpret_cast<const double2*>(v0)[i];
@@ -152,8 +153,7 @@ __global__ void __launch_bounds__(kBlock) k_predict_aabb(uint32_t n, const uint3
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      double v1;
-      flow_component<D>(integrator, mi, g[a], dt, qa[a], va[a], &q1[a], &v1);
+      flow_component<D>(integrator, mi, g[a], dt, qa[a], va[a], &q1[a], &v1[a]);
       if (D == 2) {
         // swept box of the linear motion q0 -> q1 (ball2d/Ball2DSim.cpp:565-573)
         lo[a] = fmin(q1[a], qa[a]) - ri;
@@ -163,6 +163,10 @@ __global__ void __launch_bounds__(kBlock) k_predict_aabb(uint32_t n, const uint3
         lo[a] = q1[a] - ri;
         hi[a] = q1[a] + ri;
       }
+    }
+    if (v1p) {
+#pragma unroll
+      for (int a = 0; a < D; ++a) v1p[(size_t)D * i + a] = v1[a];
     }
     if (D == 2) {
       reinterpret_cast<double2*>(q1p)[i] = make_double2(q1[0], q1[1]);
@@ -623,7 +627,7 @@ __global__ void __launch_bounds__(kBlock) k_plane_masks(uint32_t n, const double
 // Host launchers.
 // ---------------------------------------------------------------------------
 hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, const double* m, double dt,
-                               const double* g, int integrator, double* q1p) {
+                               const double* g, int integrator, double* q1p, double* v1p) {
   const uint32_t n = c->n;
   const uint32_t nblk = (n + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
@@ -631,10 +635,10 @@ hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, c
   const uint32_t* bscene = c->nscenes ? d.bscene : nullptr;
   if (c->dim == 2)
     k_predict_aabb<2><<<nblk, kBlock, 0, c->stream>>>(n, d.order, q0, v0, m, d.r, dt, g[0], g[1], g[2], integrator,
-                                                       q1p, d.tbox, d.tnar, d.tr, bscene, d.scene_shift, d.red);
+                                                       q1p, v1p, d.tbox, d.tnar, d.tr, bscene, d.scene_shift, d.red);
   else
     k_predict_aabb<3><<<nblk, kBlock, 0, c->stream>>>(n, d.order, q0, v0, m, d.r, dt, g[0], g[1], g[2], integrator,
-                                                       q1p, d.tbox, d.tnar, d.tr, bscene, d.scene_shift, d.red);
+                                                       q1p, v1p, d.tbox, d.tnar, d.tr, bscene, d.scene_shift, d.red);
   return hipGetLastError();
 }
 

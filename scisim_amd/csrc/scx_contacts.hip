@@ -70,13 +70,16 @@ struct SetupArgs {
   const uint32_t* bscene;        // ensembles: scene of each body (else nullptr)
   const uint32_t* scene_poff;    // ensembles: first plane of each scene
   const double* q0;
-  const double* v0;
+  const double* v0;  // velocity of the frames and of the CoR term (the map's choice)
   const double* w0;
   const double* r;
   const double* m;
   const double* inertia;
+  const double* px;
   const double* pn;
   double CoR;
+  double stab_dt;        // > 0: stabilization term min(0, gap(q0)) / stab_dt in the normal forcing
+  double pen_threshold;  // gaps of smaller magnitude count as 0
   uint32_t* ca;
   uint32_t* cb;
   int32_t* cplane;
@@ -116,18 +119,42 @@ __global__ void __launch_bounds__(kBlock) k_contact_setup(SetupArgs A) {
   const double minv_a = 1.0 / A.m[a];
   const double minv_b = (b != kNoBody) ? 1.0 / A.m[b] : 0.0;
   double nn[3] = {0.0, 0.0, 0.0};
+  double gap = 0.0;
   if (b != kNoBody) {
     for (int d = 0; d < D; ++d) nn[d] = A.q0[(size_t)D * a + d] - A.q0[(size_t)D * b + d];
+    if (A.stab_dt > 0.0) {
+      // evaluateGapFunction at q0 (ball2d/Constraints/BallBallConstraint.cpp:76-86: minus the
+      // radius sum; rigidbody3d/Constraints/SphereSphereConstraint.cpp:28-31: each radius)
+      double d2 = nn[0] * nn[0];
+      for (int d = 1; d < D; ++d) d2 = d2 + nn[d] * nn[d];
+      gap = (D == 2) ? sqrt(d2) - (A.r[a] + A.r[b]) : (sqrt(d2) - A.r[a]) - A.r[b];
+    }
     normalize<D>(nn);
   } else {
     for (int d = 0; d < D; ++d) nn[d] = A.pn[(size_t)D * plane + d];
+    if (A.stab_dt > 0.0) {
+      // n . (x - x_plane) - r (BallStaticPlaneConstraint.cpp:58-63, StaticPlaneSphereConstraint.cpp:31-35)
+      double s = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double dd = A.q0[(size_t)D * a + d] - A.px[(size_t)D * plane + d];
+        s = (d == 0) ? nn[0] * dd : s + nn[d] * dd;
+      }
+      gap = s - A.r[a];
+    }
   }
   double ndotv = 0.0;
   for (int d = 0; d < D; ++d) {
     const double dv = A.v0[(size_t)D * a + d] - ((b != kNoBody) ? A.v0[(size_t)D * b + d] : 0.0);
     ndotv = (d == 0) ? nn[0] * dv : ndotv + nn[d] * dv;
   }
-  A.forcing[k] = A.CoR * ndotv;
+  double forcing = A.CoR * ndotv;
+  if (A.stab_dt > 0.0) {
+    // SymplecticEulerImpactFrictionMap.cpp:309-327: g0 = min(0, g(q0)), 0 below the threshold
+    double g0 = fmin(0.0, gap);
+    if (fabs(g0) < A.pen_threshold) g0 = 0.0;
+    forcing = forcing + g0 / A.stab_dt;
+  }
+  A.forcing[k] = forcing;
   if (D == 2) {
     // basis [n, t], t = (-n_y, n_x) (BallBallConstraint.cpp:244-258, StaticPlane.cpp:10-20)
     double* B = A.basis + 4 * (size_t)k;
@@ -190,8 +217,8 @@ __global__ void __launch_bounds__(kBlock) k_contact_setup(SetupArgs A) {
   }
 }
 
-hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double mu) {
-  (void)mu;
+hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double stab_dt,
+                                double pen_threshold) {
   const uint32_t C = nbb + npl;
   const uint32_t nblk = (C + kBlock - 1) / kBlock;
   if (nblk == 0) return hipSuccess;
@@ -204,13 +231,16 @@ hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double C
   A.bscene = c->nscenes ? c->d.bscene : nullptr;
   A.scene_poff = c->d.scene_poff;
   A.q0 = c->d.q0;
-  A.v0 = c->d.v0;
+  A.v0 = c->vbasis;
   A.w0 = c->d.w0;
   A.r = c->d.r;
   A.m = c->d.m;
   A.inertia = c->d.inertia;
+  A.px = c->d.px;
   A.pn = c->d.pn;
   A.CoR = CoR;
+  A.stab_dt = stab_dt;
+  A.pen_threshold = pen_threshold;
   A.ca = c->d.ca;
   A.cb = c->d.cb;
   A.cplane = c->d.cplane;
@@ -761,12 +791,12 @@ hipError_t launch_contact_features(scx_ctx* c, uint32_t C, double mu, double vre
   DeviceBuffers& d = c->d;
   if (c->dim == 2)
     k_contact_features<2><<<nblk, kBlock, 0, c->stream>>>(C, d.ca, d.cb, d.cplane, d.basis, d.cla, d.clb, d.cwn,
-                                                           d.cwt, mu, d.cforcing, d.q0, d.v0, d.w0, d.r, d.px,
+                                                           d.cwt, mu, d.cforcing, d.q0, c->vsolve, d.w0, d.r, d.px,
                                                            d.crinit, vref, c->nscenes ? d.cmu : nullptr, d.mlp_x,
                                                            d.mlp_s);
   else
     k_contact_features<3><<<nblk, kBlock, 0, c->stream>>>(C, d.ca, d.cb, d.cplane, d.basis, d.cla, d.clb, d.cwn,
-                                                           d.cwt, mu, d.cforcing, d.q0, d.v0, d.w0, d.r, d.px,
+                                                           d.cwt, mu, d.cforcing, d.q0, c->vsolve, d.w0, d.r, d.px,
                                                            d.crinit, vref, c->nscenes ? d.cmu : nullptr, d.mlp_x,
                                                            d.mlp_s);
   return hipGetLastError();

@@ -110,6 +110,7 @@ struct DeviceBuffers {
   double* v0 = nullptr;
   double* w0 = nullptr;
   double* q1p = nullptr;     // predictor positions
+  double* v1p = nullptr;     // predictor velocities (symplectic Euler and stabilized staggerings)
   double* v2 = nullptr;      // post-impact velocity (dim*N)
   double* w2 = nullptr;      // (3D)
   // working velocities for the sweeps, per body slot: 2D [v (3) | minv], 3D
@@ -248,6 +249,12 @@ struct scx_ctx {
   int gs_blocks = 0;
   int colour_blocks = 0;
   int force_blocks = 0;  // > 0: persistent kernels use this many workgroups (capped), for tests
+  // Velocities of the current step (device, dim per body): the one the contact frames and
+  // forcing terms are evaluated at, and the one the solve starts from. Geometric map: both
+  // v0; symplectic Euler: frames at v0, solve from v0 + dt M^-1 F (the force increment
+  // folded into the start velocity); stabilized: both the predictor velocity.
+  const double* vbasis = nullptr;
+  const double* vsolve = nullptr;
   scx::SweepTuner tuner;
   scx::DeviceBuffers d;
 };
@@ -259,8 +266,10 @@ size_t scan_tmp_words(uint32_t n);
 // Records a phase boundary (HIP event on the context stream) when timing is enabled.
 void timing_mark(scx_ctx* c, const char* name);
 
+// integrator: SCX_INTEGRATOR_* or kFlowForceIncrement; v1p (may be null) receives the
+// predictor velocities.
 hipError_t launch_predict_aabb(scx_ctx* c, const double* q0, const double* v0, const double* m, double dt,
-                               const double* g, int integrator, double* q1p);
+                               const double* g, int integrator, double* q1p, double* v1p);
 hipError_t launch_flow(scx_ctx* c, uint32_t n, int integrator, const double* m, const double* g, double dt,
                        const double* q0, const double* v0, double* q1, double* v1);
 // given_boxes: bin the uploaded boxes d.aabb_min/max (scx_potential_overlaps);
@@ -273,7 +282,10 @@ hipError_t launch_pairs(scx_ctx* c, uint32_t n, int mode, int filter, uint32_t* 
 hipError_t launch_plane_masks(scx_ctx* c, uint32_t n);
 hipError_t launch_multibucket(scx_ctx* c, const unsigned long long* masks, uint32_t n, uint32_t nbuckets,
                               uint32_t* out_idx, uint32_t out_cap, uint32_t* bucket_offsets);
-hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double mu);
+// stab_dt > 0: add min(0, gap(q0)) / stab_dt to the normal forcing of every contact whose
+// gap magnitude is at least pen_threshold.
+hipError_t launch_contact_setup(scx_ctx* c, uint32_t nbb, uint32_t npl, double CoR, double stab_dt,
+                                double pen_threshold);
 // Ensembles: d.cmu[k] = mu of the scene of contact k (no-op for one scene).
 hipError_t launch_contact_mu(scx_ctx* c, uint32_t C);
 // Ensembles: per-scene contacts and residual of the last solve.

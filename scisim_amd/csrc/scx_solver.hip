@@ -1126,7 +1126,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
   const bool warm = p->warm_start != SCX_WARM_NONE;
   const uint32_t nb = (c->n + kBlock - 1) / kBlock;
   if (nb)
-    k_body_init<D><<<nb, kBlock, 0, s>>>(c->n, c->d.sid, c->d.v0, c->d.w0, c->d.m, c->d.inertia, c->d.bv64,
+    k_body_init<D><<<nb, kBlock, 0, s>>>(c->n, c->d.sid, c->vsolve, c->d.w0, c->d.m, c->d.inertia, c->d.bv64,
                                           c->d.bv32, fp32 ? 1 : 0);
   hipError_t e = hipGetLastError();
   timing_mark(c, "gs_init");
@@ -1139,7 +1139,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
     if (e != hipSuccess) return e;
   }
   if (fp32) {
-    GSArgs<float> A{c->d.c32, c->d.bv32, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff,
+    GSArgs<float> A{c->d.c32, c->d.bv32, c->d.sid, c->vsolve, c->d.w0, c->n, C, c->d.coloff,
                     reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
     A.lmap = c->d.lmap;
@@ -1158,7 +1158,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
     if (e != hipSuccess) return e;
     timing_mark(c, "gs_promote");
   }
-  GSArgs<double> B{c->d.c64, c->d.bv64, c->d.sid, c->d.v0, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
+  GSArgs<double> B{c->d.c64, c->d.bv64, c->d.sid, c->vsolve, c->d.w0, c->n, C, c->d.coloff, c->d.rbest,
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
   B.lmap = c->d.lmap;
@@ -1179,14 +1179,18 @@ hipError_t launch_solve(scx_ctx* c, uint32_t C, const uint32_t* col_off, const s
 
 hipError_t launch_finish(scx_ctx* c, uint32_t C, const scx_step_params* p) {
   hipStream_t s = c->stream;
+  // geometric map: the unconstrained map from q0 with v2 (GeometricImpactFrictionMap.cpp:441);
+  // symplectic Euler and stabilized: q1 = q0 + dt v1 with v1 the solve's result
+  // (SymplecticEulerImpactFrictionMap.cpp:405, StabilizedImpactFrictionMap.cpp:181)
+  const int integ = (p->staggering == SCX_STAGGER_GEOMETRIC) ? p->integrator : kFlowLinear;
   const uint32_t nb = (c->n + kBlock - 1) / kBlock;
   if (nb) {
     if (c->dim == 2)
       k_finish_bodies<2><<<nb, kBlock, 0, s>>>(c->n, c->d.slot_map, c->d.bv64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
-                                               p->g[2], p->integrator, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
+                                               p->g[2], integ, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
     else
       k_finish_bodies<3><<<nb, kBlock, 0, s>>>(c->n, c->d.slot_map, c->d.bv64, c->d.q0, c->d.m, p->dt, p->g[0], p->g[1],
-                                               p->g[2], p->integrator, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
+                                               p->g[2], integ, c->d.q, c->d.v, c->d.w, c->d.v2, c->d.w2);
   }
   const uint32_t nc = (C + kBlock - 1) / kBlock;
   if (nc) k_impulses_out<<<nc, kBlock, 0, s>>>(C, c->dim, c->d.perm, c->d.c64.r, c->d.calpha, c->d.cbeta, p->mu);
