@@ -232,6 +232,26 @@ int scx_warm_model_gradients(scx_ctx* ctx, uint32_t n, const float* features, co
 int scx_train_warm_model(scx_ctx* ctx, uint32_t n, const float* features, const float* targets, uint32_t epochs,
                          uint32_t batch, float lr, uint64_t seed, double* loss);
 
+/* Checkpoint and resume (the serializeSystem / deserializeSystem pair of
+ * ball2dcli/ball2d_cli.cpp:199-316 for this path, with ConstraintCache::serialize,
+ * ball2d/ConstraintCache.cpp:125-175, and the maps' serialize,
+ * GeometricImpactFrictionMap.cpp:33-44): the resident bodies (q, v, omega, r, m, inertia),
+ * the static planes, the ensemble layout, the impulse cache and the warm-start model with
+ * its Adam moments, as one byte image (a 64-byte header with magic "SCXCKPT1", version,
+ * dimension and counts, then the arrays, guarded by a hash). A context of the same
+ * dimension that loads it steps bit-identically to the one that saved it (the step counter
+ * and the step parameters stay with the caller, as the reference's CLI serializes its own).
+ * scx_save_checkpoint: buf == NULL sets *len to the size needed and returns SCX_OK; a
+ * smaller cap returns SCX_E_CAPACITY with *len set. scx_load_checkpoint replaces the
+ * context's scene (growing its buffers as needed); a malformed or corrupt image returns
+ * SCX_E_ARG and leaves the context unchanged. */
+int scx_save_checkpoint(scx_ctx* ctx, void* buf, uint64_t cap, uint64_t* len);
+int scx_load_checkpoint(scx_ctx* ctx, const void* buf, uint64_t len);
+/* Validates a checkpoint image on the host (no device, no context) and reports its
+ * contents; any output may be NULL. */
+int scx_checkpoint_info(const void* buf, uint64_t len, int32_t* dim, uint32_t* nbodies, uint32_t* nplanes,
+                        uint32_t* nscenes, uint32_t* cache_entries, int32_t* has_model);
+
 /* UnconstrainedMap::flow on host arrays (q0, v0 -> q1, v1), gravity only. */
 int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const double* m, const double* g, double dt,
                            const double* q0, const double* v0, double* q1, double* v1);

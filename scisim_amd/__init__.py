@@ -38,7 +38,8 @@ EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_po
            "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
            "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
            "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
-           "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats", "scx_set_autotune", "scx_get_autotune"]
+           "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats", "scx_set_autotune", "scx_get_autotune",
+           "scx_save_checkpoint", "scx_load_checkpoint", "scx_checkpoint_info"]
 
 MLP_PARAMS = 19075  # SCX_MLP_PARAMS
 WARM_LEARNED = 4
@@ -122,8 +123,26 @@ def lib():
         L.scx_warm_model_gradients.argtypes = [_vp, ctypes.c_uint32, _pf, _pf, _pf, _pd]
         L.scx_train_warm_model.argtypes = [_vp, ctypes.c_uint32, _pf, _pf, ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.c_float, ctypes.c_uint64, _pd]
+        L.scx_save_checkpoint.argtypes = [_vp, ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.scx_load_checkpoint.argtypes = [_vp, ctypes.c_void_p, ctypes.c_uint64]
+        L.scx_checkpoint_info.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _pi32, _pu32, _pu32, _pu32, _pu32,
+                                          _pi32]
         _lib = L
     return _lib
+
+
+def checkpoint_info(image):
+    """Validates a checkpoint image (bytes) on the host and returns its contents as a dict;
+    raises ScxError if it is malformed or corrupt."""
+    buf = ctypes.create_string_buffer(bytes(image), len(image))
+    dim, has_model = ctypes.c_int32(0), ctypes.c_int32(0)
+    n, npl, nsc, ncache = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    rc = lib().scx_checkpoint_info(buf, len(image), ctypes.byref(dim), ctypes.byref(n), ctypes.byref(npl),
+                                   ctypes.byref(nsc), ctypes.byref(ncache), ctypes.byref(has_model))
+    if rc != SCX_OK:
+        raise ScxError("scx_checkpoint_info failed (%d)" % rc)
+    return {"dim": dim.value, "bodies": n.value, "planes": npl.value, "scenes": nsc.value,
+            "cache_entries": ncache.value, "has_model": bool(has_model.value)}
 
 
 def _p(a, t=_pd):
@@ -341,6 +360,22 @@ class Context:
                  "settled": bool(st[i])} for i in range(k)]
 
     # learned warm start
+    def save_checkpoint(self):
+        """The context's state as a checkpoint image (bytes), scx_save_checkpoint."""
+        need = ctypes.c_uint64(0)
+        self._check(lib().scx_save_checkpoint(self._h, None, 0, ctypes.byref(need)), "scx_save_checkpoint")
+        buf = ctypes.create_string_buffer(need.value)
+        self._check(lib().scx_save_checkpoint(self._h, buf, need.value, ctypes.byref(need)), "scx_save_checkpoint")
+        return buf.raw[:need.value]
+
+    def load_checkpoint(self, image):
+        """Restores a checkpoint image written by save_checkpoint (scx_load_checkpoint)."""
+        info = checkpoint_info(image)
+        buf = ctypes.create_string_buffer(bytes(image), len(image))
+        self._check(lib().scx_load_checkpoint(self._h, buf, len(image)), "scx_load_checkpoint")
+        self.n = info["bodies"]
+        self.nscenes = info["scenes"]
+
     def set_warm_model(self, params):
         p = np.ascontiguousarray(params, dtype=np.float32).ravel()
         self._check(lib().scx_set_warm_model(self._h, p.ctypes.data_as(_pf), p.size), "scx_set_warm_model")

@@ -1331,4 +1331,271 @@ int scx_get_timing(scx_ctx* c, uint32_t cap, const char** names, double* ms, uin
   return SCX_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Checkpoint / resume: the state that the next step's result depends on, as one byte
+// image (the serializeSystem / deserializeSystem pair of ball2dcli/ball2d_cli.cpp:199-316
+// for this path: bodies, static geometry, the impulse cache of ConstraintCache::serialize
+// (ball2d/ConstraintCache.cpp:125-175), and this library's own ensemble layout and
+// warm-start model). Layout: a fixed header, then the arrays in the order below, each as
+// raw little-endian words; an FNV-1a hash of the payload guards against corruption.
+// The sweep order of the broad phase and the tuner's table are not saved: no result
+// depends on them.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kCkptMagic[8] = {'S', 'C', 'X', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr int kCacheEntryWords = 6;  // doubles per impulse-cache entry
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t dim;
+  uint32_t n, nplanes, nscenes, scene_planes;
+  uint32_t cache_count;
+  int32_t cache_layout;
+  uint32_t has_model, model_steps;
+  uint64_t payload;  // bytes after the header
+  uint64_t hash;     // FNV-1a of the payload
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
+
+uint64_t fnv1a(const unsigned char* p, uint64_t len) {
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t k = 0; k < len; ++k) {
+    h ^= p[k];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// Array sizes (bytes) of the payload in order, from the header.
+struct CkptLayout {
+  uint64_t sizes[16];
+  int count = 0;
+  uint64_t total = 0;
+  void add(uint64_t b) {
+    sizes[count++] = b;
+    total += b;
+  }
+};
+
+void ckpt_layout(const CkptHeader& h, CkptLayout& L) {
+  L = CkptLayout{};
+  const uint64_t D = (uint64_t)h.dim, n = h.n, P = h.nplanes, S = h.nscenes, K = h.cache_count;
+  L.add(8 * D * n);  // q
+  L.add(8 * D * n);  // v
+  L.add(8 * 3 * n);  // omega (3D; zeros in 2D)
+  L.add(8 * n);      // r
+  L.add(8 * n);      // m
+  L.add(8 * n);      // inertia (3D; zeros in 2D)
+  L.add(8 * D * P);  // plane points
+  L.add(8 * D * P);  // plane normals (normalised as stored)
+  if (S) {
+    L.add(4 * n);            // scene of each body
+    L.add(4 * (S + 1));      // scene plane offsets
+    L.add(8 * S);            // scene mu
+    L.add(8 * 3 * S);        // scene binning shifts
+  }
+  if (K) {
+    L.add(8 * K);                     // cache keys
+    L.add(8 * kCacheEntryWords * K);  // cache entries
+  }
+  if (h.has_model) {
+    L.add(4 * (uint64_t)kMlpParams);  // FP32 parameters
+    L.add(4 * (uint64_t)kMlpParams);  // Adam first moments
+    L.add(4 * (uint64_t)kMlpParams);  // Adam second moments
+  }
+}
+
+int ckpt_check(scx_ctx* c, const void* buf, uint64_t len, CkptHeader* out) {
+  if (!buf || len < sizeof(CkptHeader)) return fail(c, SCX_E_ARG, "checkpoint shorter than its header");
+  CkptHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (std::memcmp(h.magic, kCkptMagic, 8) != 0) return fail(c, SCX_E_ARG, "not an scx checkpoint (bad magic)");
+  if (h.version != kCkptVersion) return fail(c, SCX_E_ARG, "checkpoint version %u, expected %u", h.version, kCkptVersion);
+  if (h.dim != 2 && h.dim != 3) return fail(c, SCX_E_ARG, "checkpoint dimension %d", h.dim);
+  if (h.n == 0 || h.n >= (1u << 29)) return fail(c, SCX_E_ARG, "checkpoint body count %u", h.n);
+  if (h.nplanes > kMaxPlanesTotal || h.scene_planes > kMaxScenePlanes || h.cache_layout < 0 || h.cache_layout > 1 ||
+      h.has_model > 1)
+    return fail(c, SCX_E_ARG, "checkpoint header out of range");
+  CkptLayout L;
+  ckpt_layout(h, L);
+  if (h.payload != L.total || len != sizeof(CkptHeader) + L.total)
+    return fail(c, SCX_E_ARG, "checkpoint length %llu does not match its header (%llu)", (unsigned long long)len,
+                (unsigned long long)(sizeof(CkptHeader) + L.total));
+  if (fnv1a(static_cast<const unsigned char*>(buf) + sizeof(CkptHeader), L.total) != h.hash)
+    return fail(c, SCX_E_ARG, "checkpoint payload is corrupt (hash mismatch)");
+  *out = h;
+  return SCX_OK;
+}
+
+}  // namespace
+
+int scx_checkpoint_info(const void* buf, uint64_t len, int32_t* dim, uint32_t* nbodies, uint32_t* nplanes,
+                        uint32_t* nscenes, uint32_t* cache_entries, int32_t* has_model) {
+  CkptHeader h;
+  const int rc = ckpt_check(nullptr, buf, len, &h);
+  if (rc != SCX_OK) return rc;
+  if (dim) *dim = h.dim;
+  if (nbodies) *nbodies = h.n;
+  if (nplanes) *nplanes = h.nplanes;
+  if (nscenes) *nscenes = h.nscenes;
+  if (cache_entries) *cache_entries = h.cache_count;
+  if (has_model) *has_model = (int32_t)h.has_model;
+  return SCX_OK;
+}
+
+int scx_save_checkpoint(scx_ctx* c, void* buf, uint64_t cap, uint64_t* len) {
+  if (!c) return SCX_E_ARG;
+  if (!len) return fail(c, SCX_E_ARG, "scx_save_checkpoint: null length");
+  if (!c->has_bodies) return fail(c, SCX_E_STATE, "no bodies to checkpoint");
+  HIPC(hipSetDevice(c->device));
+  DeviceBuffers& d = c->d;
+  CkptHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = kCkptVersion;
+  h.dim = c->dim;
+  h.n = c->n;
+  h.nplanes = c->nplanes;
+  h.nscenes = c->nscenes;
+  h.scene_planes = c->nscenes ? c->scene_planes : 0;
+  h.cache_count = d.prev_count;
+  h.cache_layout = d.prev_layout;
+  h.has_model = d.mlp_ready ? 1 : 0;
+  h.model_steps = d.mlp_ready ? d.mlp_step : 0;
+  CkptLayout L;
+  ckpt_layout(h, L);
+  h.payload = L.total;
+  *len = sizeof(CkptHeader) + L.total;
+  if (!buf) return SCX_OK;
+  if (cap < *len) return fail(c, SCX_E_CAPACITY, "checkpoint needs %llu bytes", (unsigned long long)*len);
+  unsigned char* out = static_cast<unsigned char*>(buf) + sizeof(CkptHeader);
+  const void* src[16];
+  int k = 0;
+  src[k++] = d.q;
+  src[k++] = d.v;
+  src[k++] = d.w;
+  src[k++] = d.r;
+  src[k++] = d.m;
+  src[k++] = d.inertia;
+  src[k++] = d.px;
+  src[k++] = d.pn;
+  if (h.nscenes) {
+    src[k++] = d.bscene;
+    src[k++] = d.scene_poff;
+    src[k++] = d.scene_mu;
+    src[k++] = d.scene_shift;
+  }
+  if (h.cache_count) {
+    src[k++] = d.prev_key;
+    src[k++] = d.prev_r;
+  }
+  if (h.has_model) {
+    src[k++] = d.mlp_params;
+    src[k++] = d.mlp_m;
+    src[k++] = d.mlp_v;
+  }
+  for (int a = 0; a < L.count; ++a) {
+    if (L.sizes[a]) HIPC(hipMemcpyAsync(out, src[a], L.sizes[a], hipMemcpyDeviceToHost, c->stream));
+    out += L.sizes[a];
+  }
+  HIPC(hipStreamSynchronize(c->stream));
+  h.hash = fnv1a(static_cast<unsigned char*>(buf) + sizeof(CkptHeader), L.total);
+  std::memcpy(buf, &h, sizeof(h));
+  return SCX_OK;
+}
+
+int scx_load_checkpoint(scx_ctx* c, const void* buf, uint64_t len) {
+  if (!c) return SCX_E_ARG;
+  CkptHeader h;
+  if (ckpt_check(c, buf, len, &h) != SCX_OK) return SCX_E_ARG;
+  if (h.dim != c->dim) return fail(c, SCX_E_ARG, "checkpoint of a %dD scene, context is %dD", h.dim, c->dim);
+  HIPC(hipSetDevice(c->device));
+  DeviceBuffers& d = c->d;
+  const int D = c->dim;
+  const uint32_t n = h.n;
+  if (n > c->cap_bodies) {
+    if (alloc_bodies(c, n) != SCX_OK) return SCX_E_HIP;
+    if (alloc_entries(c, (D == 2 ? 8u : 16u) * n + 1024) != SCX_OK) return SCX_E_HIP;
+    if (alloc_contacts(c, std::max(c->cap_contacts, 8u * n)) != SCX_OK) return SCX_E_HIP;
+    if (alloc_scan_tmp(c) != SCX_OK) return SCX_E_HIP;
+  }
+  if (h.nplanes > c->cap_planes) {
+    HIPC(dalloc(&d.px, 3 * (size_t)h.nplanes));
+    HIPC(dalloc(&d.pn, 3 * (size_t)h.nplanes));
+    c->cap_planes = h.nplanes;
+  }
+  if (h.nscenes) {
+    if (h.nscenes > c->cap_scenes) {
+      HIPC(dalloc(&d.scene_poff, (size_t)h.nscenes + 1));
+      HIPC(dalloc(&d.scene_mu, h.nscenes));
+      HIPC(dalloc(&d.scene_shift, (size_t)3 * h.nscenes));
+      HIPC(dalloc(&d.scene_err, h.nscenes));
+      HIPC(dalloc(&d.scene_cnt, h.nscenes));
+      HIPC(dalloc(&d.scene_acc, (size_t)2 * h.nscenes));
+      HIPC(dalloc(&d.scene_sweeps, h.nscenes));
+      c->cap_scenes = h.nscenes;
+    }
+    HIPC(dalloc(&d.bscene, c->cap_bodies));
+  }
+  if (h.cache_count > d.cap_cache) {
+    const uint32_t want = h.cache_count + h.cache_count / 4 + 1024;
+    HIPC(dalloc(&d.prev_key, want));
+    HIPC(dalloc(&d.prev_r, (size_t)kCacheEntryWords * want));
+    d.cap_cache = want;
+  }
+  if (h.has_model && mlp_alloc(c) != SCX_OK) return SCX_E_HIP;
+  CkptLayout L;
+  ckpt_layout(h, L);
+  void* dst[16];
+  int k = 0;
+  dst[k++] = d.q;
+  dst[k++] = d.v;
+  dst[k++] = d.w;
+  dst[k++] = d.r;
+  dst[k++] = d.m;
+  dst[k++] = d.inertia;
+  dst[k++] = d.px;
+  dst[k++] = d.pn;
+  if (h.nscenes) {
+    dst[k++] = d.bscene;
+    dst[k++] = d.scene_poff;
+    dst[k++] = d.scene_mu;
+    dst[k++] = d.scene_shift;
+  }
+  if (h.cache_count) {
+    dst[k++] = d.prev_key;
+    dst[k++] = d.prev_r;
+  }
+  if (h.has_model) {
+    dst[k++] = d.mlp_params;
+    dst[k++] = d.mlp_m;
+    dst[k++] = d.mlp_v;
+  }
+  const unsigned char* in = static_cast<const unsigned char*>(buf) + sizeof(CkptHeader);
+  for (int a = 0; a < L.count; ++a) {
+    if (L.sizes[a]) HIPC(hipMemcpyAsync(dst[a], in, L.sizes[a], hipMemcpyHostToDevice, c->stream));
+    in += L.sizes[a];
+  }
+  if (h.has_model) HIPC(launch_mlp_pack(c));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->n = n;
+  c->nplanes = h.nplanes;
+  c->nscenes = h.nscenes;
+  c->scene_planes = h.scene_planes;
+  c->has_bodies = true;
+  c->order_valid = false;
+  c->last_ncontacts = 0;
+  d.prev_count = h.cache_count;
+  d.prev_layout = h.cache_layout;
+  d.features_valid = false;
+  // the model is part of the saved state: a checkpoint without one leaves none
+  d.mlp_ready = h.has_model != 0;
+  d.mlp_step = h.has_model ? h.model_steps : 0;
+  return SCX_OK;
+}
+
 }  // extern "C"
