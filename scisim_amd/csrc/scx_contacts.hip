@@ -376,7 +376,11 @@ hipError_t launch_colouring(scx_ctx* c, uint32_t C) {
   StepCounters* pctr = c->d.ctr;
   uint32_t* pbar = c->d.barrier;
   void* args[] = {&Cv, &pa, &pb, &poff, &padj, &pstate, &pctr, &pbar};
-  return hipLaunchCooperativeKernel((const void*)k_colour, dim3(blocks), dim3(kBlock), args, 0, s);
+  bool launched = false;
+  const hipError_t e2 = launch_cooperative(c, (const void*)k_colour, blocks, kBlock, args, 0, s, &launched);
+  if (e2 != hipSuccess || launched) return e2;
+  // one workgroup: its grid barrier is a workgroup barrier, no cooperative launch needed
+  return hipLaunchKernel((const void*)k_colour, dim3(1), dim3(kBlock), args, 0, s);
 }
 
 int colour_max_blocks(scx_ctx* c) {

@@ -249,6 +249,14 @@ struct scx_ctx {
   int gs_blocks = 0;
   int colour_blocks = 0;
   int force_blocks = 0;  // > 0: persistent kernels use this many workgroups (capped), for tests
+  // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
+  // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
+  // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need
+  // only co-resident workgroups, and one workgroup needs no cooperative launch at all (its
+  // grid barrier is a workgroup barrier), so after such a failure the context runs them on
+  // one workgroup with ordinary launches: slower, same results (the sweeps do not depend on
+  // the launch width).
+  bool coop_unavailable = false;
   // Velocities of the current step (device, dim per body): the one the contact frames and
   // forcing terms are evaluated at, and the one the solve starts from. Geometric map: both
   // v0; symplectic Euler: frames at v0, solve from v0 + dt M^-1 F (the force increment
@@ -316,4 +324,26 @@ hipError_t launch_mlp_gradients(scx_ctx* c, uint32_t nb, const uint32_t* idx, ui
 hipError_t launch_mlp_adam(scx_ctx* c, float lr, uint32_t step);
 int solver_max_blocks(scx_ctx* c);
 int colour_max_blocks(scx_ctx* c);
+}  // namespace scx
+
+namespace scx {
+// Launches a persistent kernel (workgroups that meet at grid_barrier) on `blocks` > 1
+// workgroups cooperatively, so that all of them are resident. If the launch is refused
+// because the device's cooperative queues are held elsewhere (hipErrorIllegalState),
+// nothing runs: *launched is false, the context is marked (scx_ctx::coop_unavailable) and
+// the caller launches the kernel on one workgroup instead (with the arguments of that
+// width).
+inline hipError_t launch_cooperative(scx_ctx* c, const void* fn, int blocks, int threads, void** args, size_t lds,
+                                     hipStream_t s, bool* launched) {
+  *launched = false;
+  if (blocks <= 1 || c->coop_unavailable) return hipSuccess;
+  const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(threads), args, (unsigned)lds, s);
+  if (e != hipErrorIllegalState) {
+    *launched = e == hipSuccess;
+    return e;
+  }
+  (void)hipGetLastError();
+  c->coop_unavailable = true;
+  return hipSuccess;
+}
 }  // namespace scx

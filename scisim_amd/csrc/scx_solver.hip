@@ -1077,10 +1077,10 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
                         !A.cscene;
   // the autotuner picks among widths around the model's (not for the LDS solve)
   const int phase = std::is_same<Real, double>::value ? 1 : 0;
-  const bool tune = c->tuner.enabled && c->force_blocks <= 0 && !lds_path;
+  const bool tune = c->tuner.enabled && c->force_blocks <= 0 && !lds_path && !c->coop_unavailable;
   const int want = tune ? tune_pick(c, phase, tune_key(D, phase, A.cscene != nullptr, A.C, nw), model, max_blocks)
                         : model;
-  const int blocks = std::max(1, std::min(max_blocks, want));
+  const int blocks = c->coop_unavailable ? 1 : std::max(1, std::min(max_blocks, want));
   A.tail_col = (blocks > 1) ? (uint32_t)tail : (uint32_t)kMaxColours;
   if (lds_path) {
     // the whole solve in one workgroup's LDS
@@ -1093,7 +1093,14 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
   void* args[] = {&A};
   hipError_t e = tune ? hipEventRecord(c->tuner.pend[phase].ev[0], c->stream) : hipSuccess;
   if (e != hipSuccess) return e;
-  e = hipLaunchCooperativeKernel((const void*)k_gs<D, Real>, dim3(blocks), dim3(kGSBlock), args, 0, c->stream);
+  bool launched = false;
+  e = launch_cooperative(c, (const void*)k_gs<D, Real>, blocks, kGSBlock, args, 0, c->stream, &launched);
+  if (e == hipSuccess && !launched) {
+    // one workgroup (chosen, or after a refused cooperative launch: see
+    // scx_ctx::coop_unavailable) sweeps every colour itself, so there is no tail split
+    A.tail_col = (uint32_t)kMaxColours;
+    e = hipLaunchKernel((const void*)k_gs<D, Real>, dim3(1), dim3(kGSBlock), args, 0, c->stream);
+  }
   if (e == hipSuccess && tune) e = hipEventRecord(c->tuner.pend[phase].ev[1], c->stream);
   return e;
 }

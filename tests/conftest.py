@@ -18,14 +18,3 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def golden_dir():
     return GOLDEN
-
-
-def pytest_collection_modifyitems(session, config, items):
-    """The C++ programs of test_gpu_cpp_interface.py run as child processes and launch
-    cooperative kernels; in our runs such launches failed with hipErrorIllegalState once the
-    test process itself had launched cooperative kernels. Those tests run first."""
-    first = [it for it in items
-             if os.path.basename(str(it.fspath)) == "test_gpu_cpp_interface.py" and it.name.startswith("test_cpp_")]
-    if first:
-        rest = [it for it in items if it not in first]
-        items[:] = first + rest
