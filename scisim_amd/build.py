@@ -60,7 +60,7 @@ def build_cpp_tests():
     src_dir = os.path.join(root, "tests", "cpp")
     out_dir = os.path.join(src_dir, "bin")
     os.makedirs(out_dir, exist_ok=True)
-    for name in ("spatial_grid_tests", "slope_tests"):
+    for name in ("spatial_grid_tests", "slope_tests", "scene_xml_tests"):
         _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(root, "include"),
               os.path.join(src_dir, name + ".cpp"), "-o", os.path.join(out_dir, name), "-L", HERE, "-lscx",
               "-Wl,-rpath," + HERE])
