@@ -65,3 +65,48 @@ def broadphase_bytes(dim, n, npairs):
     three 8-byte key/id radix passes, cell ranges, per-owner count and offset), plus 8 B
     per emitted pair."""
     return (172 if dim == 2 else 208) * n + 8 * npairs
+
+
+def kernel_trace_summary(trace_csv, kernel_prefix=None, last=0):
+    """Per-kernel statistics from a rocprofv3 --kernel-trace CSV (one row per dispatch,
+    Kernel_Name, Start_Timestamp, End_Timestamp in ns), as markdown; with kernel_prefix
+    and last > 0 also the mean duration of the last `last` dispatches whose name starts
+    with kernel_prefix (the benchmark's timed steps are the run's last GPU work)."""
+    import csv
+    rows = []
+    with open(trace_csv, newline="") as f:
+        for r in csv.DictReader(f):
+            name = r.get("Kernel_Name") or r.get("KernelName") or ""
+            t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            rows.append((t0, name, (t1 - t0) / 1000.0))
+    rows.sort()
+    stats = {}
+    for _, name, us in rows:
+        s = stats.setdefault(name, [0, 0.0, float("inf"), 0.0])
+        s[0] += 1
+        s[1] += us
+        s[2] = min(s[2], us)
+        s[3] = max(s[3], us)
+    total = sum(s[1] for s in stats.values()) or 1.0
+    out = ["| kernel | calls | avg us | min us | max us | total us | % |", "|---|---|---|---|---|---|---|"]
+    for name, (n, tot, mn, mx) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        short = name.split("(")[0].replace("void ", "").replace("scx::", "")
+        out.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f | %.1f |" % (short, n, tot / n, mn, mx, tot, 100.0 * tot / total))
+    if kernel_prefix and last > 0:
+        sel = [us for _, name, us in rows
+               if name.replace("void ", "").replace("scx::", "").startswith(kernel_prefix)][-last:]
+        if sel:
+            out.append("")
+            out.append("Last %d dispatches of `%s*` (the timed steps): mean %.1f us, total %.1f us." %
+                       (len(sel), kernel_prefix, sum(sel) / len(sel), sum(sel)))
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="Kernel statistics of a rocprofv3 --kernel-trace CSV")
+    ap.add_argument("trace_csv")
+    ap.add_argument("--kernel", default=None, help="name prefix of the kernel whose last dispatches are averaged")
+    ap.add_argument("--last", type=int, default=0, help="number of last dispatches to average")
+    a = ap.parse_args()
+    print(kernel_trace_summary(a.trace_csv, a.kernel, a.last))

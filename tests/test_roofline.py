@@ -44,3 +44,20 @@ def test_broadphase_bytes():
     assert rl.broadphase_bytes(2, 1, 0) == 52 + 48 + 16 + 48 + 8 == 172
     assert rl.broadphase_bytes(3, 1, 0) == 72 + 48 + 12 + 68 + 8 == 208
     assert rl.broadphase_bytes(2, 10, 3) == 1720 + 24
+
+
+def test_kernel_trace_summary_averages_the_last_dispatches(tmp_path):
+    import csv
+    p = tmp_path / "trace.csv"
+    with open(p, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kind", "Agent_Id", "Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        t = 0
+        for k in range(6):
+            w.writerow(["KERNEL_DISPATCH", 1, "void scx::k_gs<3, float>(scx::GSArgs<float>)", t, t + 1000 * (k + 1)])
+            t += 10000
+            w.writerow(["KERNEL_DISPATCH", 1, "k_pairs", t, t + 500])
+            t += 1000
+    out = rl.kernel_trace_summary(str(p), "k_gs", 2)
+    assert "| `k_gs<3, float>` | 6 | 3.5 | 1.0 | 6.0 | 21.0 |" in out
+    assert "Last 2 dispatches of `k_gs*` (the timed steps): mean 5.5 us" in out
