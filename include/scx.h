@@ -177,6 +177,14 @@ int scx_run(scx_ctx* ctx, const scx_step_params* p, uint32_t nsteps, scx_step_st
  * ((dim-1) per contact), colour. Any output pointer may be NULL. */
 int scx_get_contacts(scx_ctx* ctx, uint32_t cap, int32_t* body0, int32_t* body1, int32_t* plane, double* bases,
                      double* alpha, double* beta, int32_t* colour, uint32_t* ncontacts);
+/* Last step's contacts in the layout of the reference's force output
+ * (ImpactFrictionMap::exportConstraintForcesToBinaryFile, scisim/ConstrainedMaps/ImpactFrictionMap.cpp:58-163,
+ * datasets collision_indices, collision_points, collision_normals, collision_forces):
+ * indices 2 per contact (body0, then body1 or -plane - 2 for a static plane), points dim per
+ * contact (the world-space contact point at q0), normals dim, forces dim (the impulse
+ * alpha n + sum_k beta_k t_k, as the reference writes it). All arrays must be given. */
+int scx_get_contact_forces(scx_ctx* ctx, uint32_t cap, int32_t* indices, double* points, double* normals,
+                           double* forces, uint32_t* ncontacts);
 /* Last step's post-impact velocity v2 (dim*n; 3D: plus omega2 3n if not NULL). Under the
  * symplectic Euler and stabilized staggerings this is the step's final velocity v1. */
 int scx_get_v2(scx_ctx* ctx, double* v2, double* omega2);

@@ -64,6 +64,8 @@ def lib():
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_clone.restype = ctypes.c_void_p
+        L.orc_clone.argtypes = [ctypes.c_void_p]
         L.orc_step.restype = ctypes.c_int
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _pd, _pd, _pd, _pd, _pd, _pd, ctypes.c_int, _pd, _pd,
                                ctypes.POINTER(OrcParams), ctypes.POINTER(OrcStats)]
@@ -205,6 +207,14 @@ class Sim:
         self.pn = _f64(planes_n).ravel().copy()
         self.nplanes = self.px.size // dim
         self._ctx = lib().orc_create(dim)
+
+    def clone(self):
+        """An independent copy: state, impulse cache and last step."""
+        c = Sim.__new__(Sim)
+        c.__dict__.update({k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in self.__dict__.items()
+                           if k != "_ctx"})
+        c._ctx = lib().orc_clone(self._ctx)
+        return c
 
     def __del__(self):
         if getattr(self, "_ctx", None) and _lib is not None:

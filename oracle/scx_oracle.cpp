@@ -549,6 +549,8 @@ struct Solver {
     // (Ball2DSobogusInterface.cpp:142 with f = -M v0)
     for (size_t i = 0; i < (size_t)B.n * DOF; ++i) v[i] = v0[i];
     for (size_t ci = 0; ci < C; ++ci) apply((int)ci, r + D * ci, v);
+    // GSIter as So-bogus reports it (scripts_include/sobogus_patch.patch:67-70): one past the
+    // limit when the limit is reached
     *sweeps = it;
     return err_best;
   }
@@ -911,6 +913,8 @@ orc_ctx* orc_create(int dim) {
   return c;
 }
 void orc_destroy(orc_ctx* c) { delete c; }
+// A copy of a context (impulse cache and last step), for stepping the same state twice.
+orc_ctx* orc_clone(const orc_ctx* c) { return new orc_ctx(*c); }
 
 // Gap function at q (evaluateGapFunction): ball2d/Constraints/BallBallConstraint.cpp:76-86,
 // BallStaticPlaneConstraint.cpp:58-63, rigidbody3d/Constraints/SphereSphereConstraint.cpp:28-31,

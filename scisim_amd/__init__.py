@@ -39,7 +39,7 @@ EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_po
            "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
            "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
            "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats", "scx_set_autotune", "scx_get_autotune",
-           "scx_save_checkpoint", "scx_load_checkpoint", "scx_checkpoint_info"]
+           "scx_save_checkpoint", "scx_load_checkpoint", "scx_checkpoint_info", "scx_get_contact_forces"]
 
 MLP_PARAMS = 19075  # SCX_MLP_PARAMS
 WARM_LEARNED = 4
@@ -109,6 +109,7 @@ def lib():
         L.scx_run.argtypes = [_vp, ctypes.POINTER(ScxStepParams), ctypes.c_uint32, ctypes.POINTER(ScxStepStats)]
         L.scx_get_contacts.argtypes = [_vp, ctypes.c_uint32, _pi32, _pi32, _pi32, _pd, _pd, _pd, _pi32, _pu32]
         L.scx_get_v2.argtypes = [_vp, _pd, _pd]
+        L.scx_get_contact_forces.argtypes = [_vp, ctypes.c_uint32, _pi32, _pd, _pd, _pd, _pu32]
         L.scx_friction_solve.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, _pd, _pd, ctypes.POINTER(ScxContactSet), _d,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32, _pd, _pd, _pd, _pd, _pd,
                                          _pi32, _pd, _pu32]
@@ -286,6 +287,22 @@ class Context:
                                            _p(alpha), _p(beta), _p(col, _pi32), ctypes.byref(nc)),
                     "scx_get_contacts")
         return {"a": a, "b": b, "plane": pl, "basis": basis, "alpha": alpha, "beta": beta, "colour": col}
+
+    def contact_forces(self):
+        """The last step's contacts as the reference's force output datasets
+        (scx_get_contact_forces), shaped as the reference writes them (one column per
+        contact): collision_count, collision_indices (2, C; a static plane as -plane - 2),
+        collision_points, collision_normals, collision_forces (dim, C)."""
+        C = self.contacts()["a"].size
+        D = self.dim
+        idx = np.empty((C, 2), np.int32)
+        pts, nrm, frc = np.empty((C, D)), np.empty((C, D)), np.empty((C, D))
+        cnt = ctypes.c_uint32(0)
+        self._check(lib().scx_get_contact_forces(self._h, C, _p(idx, _pi32), _p(pts), _p(nrm), _p(frc),
+                                                 ctypes.byref(cnt)), "scx_get_contact_forces")
+        return {"collision_count": int(cnt.value), "collision_indices": np.ascontiguousarray(idx.T),
+                "collision_points": np.ascontiguousarray(pts.T), "collision_normals": np.ascontiguousarray(nrm.T),
+                "collision_forces": np.ascontiguousarray(frc.T)}
 
     def v2(self):
         v2 = np.empty(self.dim * self.n)

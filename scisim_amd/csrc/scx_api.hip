@@ -31,6 +31,8 @@ namespace {
 
 const char* kVersion = "scx 0.1 (gfx950)";
 
+
+
 int fail(scx_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -843,6 +845,7 @@ int scx_get_contacts(scx_ctx* c, uint32_t cap, int32_t* body0, int32_t* body1, i
   const int D = c->dim;
   hipStream_t s = c->stream;
   if (C == 0) return SCX_OK;
+  HIPC(hipStreamSynchronize(s));  // the synchronous copies below do not wait for the context stream
   std::vector<uint32_t> tmp(C);
   if (body0) {
     HIPC(hipMemcpy(tmp.data(), c->d.ca, sizeof(uint32_t) * C, hipMemcpyDeviceToHost));
@@ -861,6 +864,91 @@ int scx_get_contacts(scx_ctx* c, uint32_t cap, int32_t* body0, int32_t* body1, i
     HIPC(hipStreamSynchronize(s));
     for (uint32_t k = 0; k < C; ++k) colour[k] &= 0xff;
   }
+  HIPC(hipStreamSynchronize(s));
+  return SCX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The reference's force output of one contact (ImpactFrictionMap::exportConstraintForcesToBinaryFile,
+// scisim/ConstrainedMaps/ImpactFrictionMap.cpp:58-163): body indices with a static plane as
+// -plane - 2 (:58-66); the world-space contact point at q0 (2D x_a - r_a n:
+// ball2d/Constraints/BallBallConstraint.cpp:219-222, BallStaticPlaneConstraint.cpp:171-174;
+// 3D sphere pairs the point x_a + r_a / (r_a + r_b) (x_b - x_a) of the narrow phase,
+// rigidbody3d/RigidBody3DSim.cpp:803-809, plane x_a - r_a n:
+// StaticPlaneSphereConstraint.cpp:314-317); the normal; the impulse alpha n + sum beta_k t_k.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_contact_forces(uint32_t C, const uint32_t* __restrict__ ca,
+                                                           const uint32_t* __restrict__ cb,
+                                                           const int32_t* __restrict__ cplane,
+                                                           const double* __restrict__ basis,
+                                                           const double* __restrict__ alpha,
+                                                           const double* __restrict__ beta,
+                                                           const double* __restrict__ q0, const double* __restrict__ r,
+                                                           int32_t* __restrict__ idx, double* __restrict__ pts,
+                                                           double* __restrict__ nrm, double* __restrict__ frc) {
+#pragma clang fp contract(off)
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= C) return;
+  const uint32_t a = ca[k], b = cb[k];
+  const double* B = basis + (size_t)D * D * k;
+  idx[2 * (size_t)k] = (int32_t)a;
+  idx[2 * (size_t)k + 1] = (b == kNoBody) ? -cplane[k] - 2 : (int32_t)b;
+  const double al = alpha[k];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double xa = q0[(size_t)D * a + d];
+    double p;
+    if (D == 3 && b != kNoBody) {
+      const double f = r[a] / (r[a] + r[b]);
+      p = xa + f * (q0[(size_t)D * b + d] - xa);
+    } else {
+      p = xa - r[a] * B[d];
+    }
+    pts[(size_t)D * k + d] = p;
+    nrm[(size_t)D * k + d] = B[d];
+    double fo = al * B[d];
+    for (int t = 1; t < D; ++t) fo = fo + beta[(size_t)(D - 1) * k + t - 1] * B[(size_t)D * t + d];
+    frc[(size_t)D * k + d] = fo;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int scx_get_contact_forces(scx_ctx* c, uint32_t cap, int32_t* indices, double* points, double* normals,
+                           double* forces, uint32_t* ncontacts) {
+  if (!c) return SCX_E_ARG;
+  HIPC(hipSetDevice(c->device));
+  const uint32_t C = c->last_ncontacts;
+  if (ncontacts) *ncontacts = C;
+  if (C > cap) return fail(c, SCX_E_CAPACITY, "need room for %u contacts", C);
+  if (C == 0) return SCX_OK;
+  if (!indices || !points || !normals || !forces) return fail(c, SCX_E_ARG, "scx_get_contact_forces: null array");
+  const int D = c->dim;
+  hipStream_t s = c->stream;
+  DeviceBuffers& d = c->d;
+  // staging in the contact-capacity buffers that the next step overwrites anyway: the
+  // records' FP64 arrays (3 per contact) hold points, normals and forces, pmask the indices
+  double* pts = d.c64.n;
+  double* nrm = d.c64.s;
+  double* frc = d.c64.r;
+  int32_t* idx = reinterpret_cast<int32_t*>(d.pmask);
+  const uint32_t nb = (C + kBlock - 1) / kBlock;
+  if (D == 2)
+    k_contact_forces<2><<<nb, kBlock, 0, s>>>(C, d.ca, d.cb, d.cplane, d.basis, d.calpha, d.cbeta, d.q0, d.r, idx,
+                                               pts, nrm, frc);
+  else
+    k_contact_forces<3><<<nb, kBlock, 0, s>>>(C, d.ca, d.cb, d.cplane, d.basis, d.calpha, d.cbeta, d.q0, d.r, idx,
+                                               pts, nrm, frc);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(indices, idx, sizeof(int32_t) * 2 * C, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(points, pts, sizeof(double) * D * C, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(normals, nrm, sizeof(double) * D * C, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(forces, frc, sizeof(double) * D * C, hipMemcpyDeviceToHost, s));
   HIPC(hipStreamSynchronize(s));
   return SCX_OK;
 }
@@ -1097,6 +1185,7 @@ int scx_potential_overlaps(scx_ctx* c, uint32_t n, const double* aabb_min, const
   }
   return fail(c, SCX_E_CAPACITY, "grid entry buffer kept overflowing");
 }
+
 
 // ---------------------------------------------------------------------------
 // Learned warm start: model I/O, training samples, training and gradients.

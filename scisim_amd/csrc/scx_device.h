@@ -90,21 +90,20 @@ __device__ __forceinline__ void wait_stores() {
 }
 
 // Grid-wide barrier for persistent kernels (launched cooperatively, so that every
-// workgroup is resident). Two levels: workgroups arrive on their group's counter
-// (groups of kBarrierGroup, one 128-byte line each); the last of a group arrives on the
-// top counter; the last group bumps the generation word that everyone else polls.
-// Counters return to zero, so the state is reusable without a reset.
-// Fences: every wave first waits for its own stores to complete (vmcnt 0; a workgroup
-// barrier alone does not wait for them on CDNA), then one
-// wave per workgroup fences at agent scope before arriving (the L2 write-back that
-// publishes the workgroup's stores to the other XCDs) and after leaving (the cache
-// invalidation that drops stale lines for the whole CU). An agent-scope fence costs a
-// cache write-back or invalidation per wave that issues it, so one wave instead of all
-// of them makes the barrier several times cheaper on a full chip.
-// Layout (u32 words): [0] generation, [32] top counter, [64 + 32 g] group g counter.
-constexpr uint32_t kBarrierGroup = 16;
-constexpr uint32_t kBarrierMaxGroups = 64;
-constexpr uint32_t kBarrierWords = 64 + 32 * kBarrierMaxGroups;
+// workgroup is resident). One wave per workgroup arrives on a counter with an
+// agent-scope release-acquire increment; the last arriver resets the counter and bumps
+// the generation word, which the others poll with plain atomic loads and follow with one
+// agent-scope acquire fence. The counter returns to zero, so the state is reusable across
+// launches of any width without a reset. Every wave first waits for its own stores to
+// complete (vmcnt 0; a workgroup barrier alone does not wait for them on CDNA). The
+// agent-scope release (the L2 write-back that publishes the workgroup's stores to the
+// other XCDs) and acquire (the invalidation that drops stale lines for the whole CU) are
+// each issued once per workgroup: on MI355X their cost, not the atomics, is what grows
+// with the workgroup count. Measured with 512-thread workgroups: 1.5 us at 8 workgroups,
+// 2.9 us at 64, 4.8 us at 128 (a two-level tree of counters with a fence on each side:
+// 3.4, 7.1 and 14.0 us).
+// Layout (u32 words): [0] generation, [32] arrival counter (separate 128-byte lines).
+constexpr uint32_t kBarrierWords = 64;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
   // one workgroup: its waves share the CU's L1, a workgroup barrier orders their
@@ -116,41 +115,30 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
   wait_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t nblocks = gridDim.x;
-    const uint32_t group = blockIdx.x / kBarrierGroup;
-    const uint32_t ngroups = (nblocks + kBarrierGroup - 1) / kBarrierGroup;
-    const uint32_t gsize = min(kBarrierGroup, nblocks - group * kBarrierGroup);
     uint32_t* gen = bar;
-    uint32_t* top = bar + 32;
-    uint32_t* gcnt = bar + 64 + 32 * group;
-    const uint32_t g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    bool last = false;
-    if (__hip_atomic_fetch_add(gcnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gsize - 1) {
-      __hip_atomic_store(gcnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1) {
-        __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = true;
-      }
-    }
-    if (last) {
+    uint32_t* cnt = bar + 32;
+    // the generation before this workgroup arrives (it cannot advance before then); the
+    // release increment below orders this load before the arrival
+    const uint32_t g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+      while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __threadfence();
   }
   __syncthreads();
 }
 
 // Time (us) of one barrier interval of a persistent launch over `width` items with `b`
 // workgroups of `threads`, from timings on MI355X. One workgroup: ~0.2 us barrier, ~1.9 us
-// per pass (caches stay warm). Several: ~0.6 us + 0.1 us per workgroup of barrier, ~4.5 us
+// per pass (caches stay warm). Several: ~1.2 us + 0.035 us per workgroup of barrier, ~4.5 us
 // for the first pass after the cache invalidation and ~3.5 us for each further one.
 inline double interval_cost(uint32_t width, int threads, int b) {
   double passes = (double)((width + (uint64_t)threads * b - 1) / ((uint64_t)threads * b));
   if (passes < 1.0) passes = 1.0;
-  return (b == 1) ? 0.2 + 1.9 * passes : (0.6 + 0.1 * b) + 4.5 + 3.5 * (passes - 1.0);
+  return (b == 1) ? 0.2 + 1.9 * passes : (1.2 + 0.035 * b) + 4.5 + 3.5 * (passes - 1.0);
 }
 
 // Workgroups (a power of two up to max_blocks) minimising the summed time of barrier

