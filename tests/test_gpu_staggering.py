@@ -96,8 +96,15 @@ def _lockstep(ctx, sim, prm, warm, nsteps, tag, stag_kw):
             continue
         q, v, w = ctx.get_state()
         amax = max(1e-300, np.abs(oc["alpha"]).max())
-        e = {"alpha": rel(gc["alpha"], oc["alpha"]), "beta": rel(gc["beta"], oc["beta"], amax),
-             "v2": rel(ctx.v2()[0], sim.v2()), "v1": rel(v, sim.v), "q1": rel(q, sim.q, 1.0)}
+        e = {"v2": rel(ctx.v2()[0], sim.v2()), "v1": rel(v, sim.v), "q1": rel(q, sim.q, 1.0)}
+        # impulses are compared where both solves converged: at the sweep limit the best
+        # iterates of a statically indeterminate contact set (a body resting on several
+        # contacts) may differ along the impulses that leave every velocity unchanged
+        imp = {"alpha": rel(gc["alpha"], oc["alpha"]), "beta": rel(gc["beta"], oc["beta"], amax)}
+        if st["succeeded"] and so["succeeded"]:
+            e.update(imp)
+        else:
+            _say(tag, "step", k, "at the sweep limit: impulse distance (reported only)", imp)
         if sim.dim == 3:
             e["omega"] = rel(w, sim.w)
         for key, x in e.items():
