@@ -136,6 +136,19 @@ void scx_destroy(scx_ctx* ctx);
 int scx_potential_overlaps(scx_ctx* ctx, uint32_t n, const double* aabb_min, const double* aabb_max,
                            uint64_t cap, uint32_t* pairs_out, uint64_t* npairs);
 
+/* scx_potential_overlaps without any allocation. No counterpart in the reference: this is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and results as
+ * scx_potential_overlaps, bit for bit, but the broad phase's working memory is the library's
+ * file-scope scratch array (40 MiB of device memory on each device the library runs on, held
+ * until the process ends) instead of the context's buffers, and nothing is allocated: at most
+ * 65,536 boxes, 16 (3D) or 8 (2D) cell entries per box and 1,048,576 pairs; beyond these it
+ * returns SCX_E_CAPACITY (with *npairs set when the pair count is the limit). Every call on a
+ * device shares that device's array: calls that overlap in time on one device (from several
+ * threads or contexts) race and must be serialised by the caller. */
+int scx_potential_overlaps_noalloc(scx_ctx* ctx, uint32_t n, const double* aabb_min, const double* aabb_max,
+                                   uint64_t cap, uint32_t* pairs_out, uint64_t* npairs);
+
 /* Scene state, in the reference's layouts: q, v = dim*n interleaved (ball2d q, v;
  * rigidbody3d first 3n entries of q and v); omega = 3n angular velocities (3D; NULL = 0);
  * r radii; m masses; inertia = isotropic sphere inertia (3D; NULL = 2/5 m r^2). */

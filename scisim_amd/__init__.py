@@ -39,7 +39,7 @@ EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_po
            "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
            "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
            "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats", "scx_set_autotune", "scx_get_autotune",
-           "scx_save_checkpoint", "scx_load_checkpoint", "scx_checkpoint_info", "scx_get_contact_forces"]
+           "scx_save_checkpoint", "scx_load_checkpoint", "scx_checkpoint_info", "scx_get_contact_forces", "scx_potential_overlaps_noalloc"]
 
 MLP_PARAMS = 19075  # SCX_MLP_PARAMS
 WARM_LEARNED = 4
@@ -100,6 +100,8 @@ def lib():
         L.scx_destroy.argtypes = [_vp]
         L.scx_potential_overlaps.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, ctypes.c_uint64, _pu32,
                                              ctypes.POINTER(ctypes.c_uint64)]
+        L.scx_potential_overlaps_noalloc.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, ctypes.c_uint64, _pu32,
+                                                     ctypes.POINTER(ctypes.c_uint64)]
         L.scx_set_bodies.argtypes = [_vp, ctypes.c_uint32, _pd, _pd, _pd, _pd, _pd, _pd]
         L.scx_set_planes.argtypes = [_vp, ctypes.c_uint32, _pd, _pd]
         L.scx_set_scenes.argtypes = [_vp, ctypes.c_uint32, _pu32, _pu32, _pd]

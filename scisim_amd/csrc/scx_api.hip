@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "scx_device.h"
@@ -27,11 +28,37 @@ void scx::timing_mark(scx_ctx* c, const char* name) {
   c->event_names.push_back(name);
 }
 
+// Scratch of scx_potential_overlaps_noalloc: file-scope device memory, one copy on every
+// device the library runs on, held until the process ends. In host code `scratch` is only a
+// stand-in for the symbol; its device address on each device comes from
+// hipGetSymbolAddress (scratch_address below) and is the only way host code refers to it.
+constexpr size_t kNoAllocScratchBytes = (size_t)40 << 20;  // 40 MiB
+constexpr uint32_t kNoAllocMaxBoxes = 1u << 16;            // 65,536 boxes
+constexpr uint32_t kNoAllocMaxPairs = 1u << 20;            // 1,048,576 pairs
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 const char* kVersion = "scx 0.1 (gfx950)";
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
 
+// The scratch array's address on `device`, which must be current; looked up once per device.
+hipError_t scratch_address(int device, unsigned char** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
+}
 
 int fail(scx_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -546,6 +573,10 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     if (const char* env = getenv("SCX_FORCE_PERSISTENT_BLOCKS")) c->force_blocks = std::max(0, atoi(env));
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
+      break;
+    }
+    if ((e = scratch_address(cfg->device, &c->scratch)) != hipSuccess) {
+      rc = fail(c, SCX_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
       break;
     }
   } while (0);
@@ -1186,6 +1217,120 @@ int scx_potential_overlaps(scx_ctx* c, uint32_t n, const double* aabb_min, const
   return fail(c, SCX_E_CAPACITY, "grid entry buffer kept overflowing");
 }
 
+}  // extern "C"
+
+namespace {
+
+// Interleaves the pair lists into (a, b) pairs for one copy to the caller.
+__global__ void __launch_bounds__(kBlock) k_interleave_pairs(uint32_t P, const uint32_t* __restrict__ a,
+                                                             const uint32_t* __restrict__ b, uint32_t* __restrict__ out) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= P) return;
+  out[2 * (size_t)k] = a[k];
+  out[2 * (size_t)k + 1] = b[k];
+}
+
+// Bump allocator over the scratch array (256-byte aligned pieces).
+struct Carve {
+  unsigned char* base;
+  size_t used = 0;
+  template <typename T>
+  T* take(size_t count) {
+    T* p = reinterpret_cast<T*>(base + used);
+    used += (sizeof(T) * std::max<size_t>(count, 1) + 255) / 256 * 256;
+    return p;
+  }
+};
+
+// The context's broad-phase buffers and capacities, restored when the no-allocation
+// entry point returns (it points them at the scratch array for the call).
+struct BroadphaseSwap {
+  scx_ctx* c;
+  DeviceBuffers saved;
+  uint32_t cap_cells, cap_entries;
+  bool order_valid;
+  explicit BroadphaseSwap(scx_ctx* ctx)
+      : c(ctx), saved(ctx->d), cap_cells(ctx->cap_cells), cap_entries(ctx->cap_entries), order_valid(ctx->order_valid) {}
+  ~BroadphaseSwap() {
+    c->d = saved;
+    c->cap_cells = cap_cells;
+    c->cap_entries = cap_entries;
+    c->order_valid = order_valid;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int scx_potential_overlaps_noalloc(scx_ctx* c, uint32_t n, const double* aabb_min, const double* aabb_max,
+                                   uint64_t cap, uint32_t* pairs_out, uint64_t* npairs) {
+  if (!c) return SCX_E_ARG;
+  if (npairs) *npairs = 0;
+  if (n == 0) return SCX_OK;
+  if (!aabb_min || !aabb_max) return fail(c, SCX_E_ARG, "null box arrays");
+  if (n > kNoAllocMaxBoxes)
+    return fail(c, SCX_E_CAPACITY, "scx_potential_overlaps_noalloc: at most %u boxes (got %u)", kNoAllocMaxBoxes, n);
+  if (!c->scratch) return fail(c, SCX_E_STATE, "no scratch address for device %d", c->device);
+  HIPC(hipSetDevice(c->device));
+  const int D = c->dim;
+  hipStream_t s = c->stream;
+  BroadphaseSwap guard(c);
+  DeviceBuffers& d = c->d;
+  // the broad phase's working set, carved from the scratch array (same sizes as a context
+  // of n bodies: 2n + 1024 cells, 8n / 16n + 1024 cell entries)
+  const uint32_t cap_cells = 2 * n + 1024;
+  const uint32_t cap_entries = (D == 2 ? 8u : 16u) * n + 1024;
+  Carve sc{c->scratch};
+  d.aabb_min = sc.take<double>((size_t)D * n);
+  d.aabb_max = sc.take<double>((size_t)D * n);
+  d.red = sc.take<double>(9 * (size_t)((n + kBlock - 1) / kBlock + 1));
+  d.grid = sc.take<GridParams>(1);
+  d.ctr = sc.take<StepCounters>(1);
+  d.cells = sc.take<uint32_t>(4 * ((size_t)cap_cells + 1));
+  d.ent = sc.take<uint32_t>(cap_entries);
+  d.order = sc.take<uint32_t>(n);
+  d.order_next = sc.take<uint32_t>(n);
+  d.tbox = sc.take<float4>(2 * (size_t)n);
+  d.tnar = sc.take<double4>(2 * (size_t)n);
+  d.tr = sc.take<double>(n);
+  d.pcount = sc.take<uint32_t>((size_t)n + 1);
+  d.scan_tmp = sc.take<uint32_t>(scan_tmp_words(std::max<uint32_t>(cap_cells + 1, kNoAllocMaxPairs)) + 4);
+  d.pa = sc.take<uint32_t>(kNoAllocMaxPairs);
+  d.pb = sc.take<uint32_t>(kNoAllocMaxPairs);
+  uint32_t* pout = sc.take<uint32_t>(2 * (size_t)kNoAllocMaxPairs);
+  if (sc.used > kNoAllocScratchBytes) return fail(c, SCX_E_CAPACITY, "scratch layout exceeds the scratch array");
+  c->cap_cells = cap_cells;
+  c->cap_entries = cap_entries;
+  c->order_valid = false;
+  HIPC(hipMemcpyAsync(d.aabb_min, aabb_min, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemcpyAsync(d.aabb_max, aabb_max, sizeof(double) * D * n, hipMemcpyHostToDevice, s));
+  HIPC(hipMemsetAsync(d.ctr, 0, sizeof(StepCounters), s));
+  HIPC(launch_grid_build(c, n, true));
+  HIPC(launch_pairs(c, n, 0, 0, nullptr, nullptr, 0));
+  HIPC(launch_exclusive_scan(d.pcount, d.pcount, n, d.scan_tmp, s));
+  HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(d.host_u32, d.pcount + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if ((d.ctr_host->overflow & 1u) || d.ctr_host->nentries > cap_entries)
+    return fail(c, SCX_E_CAPACITY, "scx_potential_overlaps_noalloc: %llu cell entries exceed its %u",
+                (unsigned long long)d.ctr_host->nentries, cap_entries);
+  const uint32_t P = d.host_u32[0];
+  if (npairs) *npairs = P;
+  if (P > kNoAllocMaxPairs)
+    return fail(c, SCX_E_CAPACITY, "scx_potential_overlaps_noalloc: %u pairs exceed its %u", P, kNoAllocMaxPairs);
+  if (P > cap) return fail(c, SCX_E_CAPACITY, "need room for %u pairs", P);
+  if (P == 0) return SCX_OK;
+  if (!pairs_out) return fail(c, SCX_E_ARG, "null pair array");
+  HIPC(launch_pairs(c, n, 1, 0, d.pa, d.pb, kNoAllocMaxPairs));
+  k_interleave_pairs<<<(P + kBlock - 1) / kBlock, kBlock, 0, s>>>(P, d.pa, d.pb, pout);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(pairs_out, pout, sizeof(uint32_t) * 2 * (size_t)P, hipMemcpyDeviceToHost, s));
+  HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+  HIPC(hipStreamSynchronize(s));
+  if (d.ctr_host->overflow & 2u) return fail(c, SCX_E_CAPACITY, "pair buffer overflow");
+  return SCX_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Learned warm start: model I/O, training samples, training and gradients.

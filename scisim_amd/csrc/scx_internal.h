@@ -257,6 +257,9 @@ struct scx_ctx {
   // one workgroup with ordinary launches: slower, same results (the sweeps do not depend on
   // the launch width).
   bool coop_unavailable = false;
+  // Device address of the library's file-scope scratch array on this context's device
+  // (scx_potential_overlaps_noalloc), looked up once per device at scx_create.
+  unsigned char* scratch = nullptr;
   // Velocities of the current step (device, dim per body): the one the contact frames and
   // forcing terms are evaluated at, and the one the solve starts from. Geometric map: both
   // v0; symplectic Euler: frames at v0, solve from v0 + dt M^-1 F (the force increment
