@@ -211,6 +211,7 @@ int alloc_contacts(scx_ctx* c, uint32_t C) {
   c->cap_perm = std::max<uint32_t>(C, c->cap_bodies);
   HIPC(dalloc(&d.perm, c->cap_perm));
   HIPC(dalloc(&d.rbest, 3 * (size_t)C));
+  HIPC(dalloc(&d.act_idx, 2 * (size_t)C));
   if (alloc_soa(c, d.c64, C) != SCX_OK) return SCX_E_HIP;
   if (alloc_soa(c, d.c32, C) != SCX_OK) return SCX_E_HIP;
   HIPC(dalloc(&d.pmask, std::max<size_t>(c->cap_bodies, C)));
@@ -244,6 +245,8 @@ void free_all(scx_ctx* c) {
   dfree(&d.cscene);
   dfree(&d.scene_acc);
   dfree(&d.scene_sweeps);
+  dfree(&d.act_idx);
+  dfree(&d.act_part);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -571,7 +574,11 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     // a fixed workgroup count for the persistent kernels, so that tests can run the
     // device-wide barrier paths on small problems
     if (const char* env = getenv("SCX_FORCE_PERSISTENT_BLOCKS")) c->force_blocks = std::max(0, atoi(env));
-    if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess) {
+    // SCX_ENSEMBLE_COMPACT=0: ensembles skip frozen contacts one by one instead of sweeping
+    // compacted lists (the same iterates bit for bit; for A/B measurements and tests)
+    if (const char* env = getenv("SCX_ENSEMBLE_COMPACT")) c->ens_compact = atoi(env) != 0;
+    if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
+        (e = dalloc(&d.act_part, (size_t)kMaxColours * std::max(c->gs_blocks, 1))) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
       break;
     }

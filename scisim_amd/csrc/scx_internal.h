@@ -139,6 +139,8 @@ struct DeviceBuffers {
   uint32_t* cscene = nullptr;      // per contact in colour order: its scene (cap_contacts)
   unsigned long long* scene_acc = nullptr;  // 2 S: per-scene residual bits of the sweeps
   uint32_t* scene_sweeps = nullptr;         // S: sweep at which each scene met the tolerance
+  uint32_t* act_idx = nullptr;   // 2 x cap_contacts: compacted lists of the unfrozen scenes' contacts
+  uint32_t* act_part = nullptr;  // kMaxColours x gs_blocks: per-colour per-workgroup counts
   // broad phase
   double* aabb_min = nullptr;  // dim*N
   double* aabb_max = nullptr;
@@ -249,6 +251,7 @@ struct scx_ctx {
   int gs_blocks = 0;
   int colour_blocks = 0;
   int force_blocks = 0;  // > 0: persistent kernels use this many workgroups (capped), for tests
+  bool ens_compact = true;  // ensembles: sweep compacted lists of the unfrozen scenes' contacts
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
   // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
   // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need

@@ -204,6 +204,11 @@ struct GSArgs {
   unsigned long long* scene_acc;  // 2 x nscenes
   uint32_t* scene_sweeps;         // nscenes
   uint32_t nscenes;
+  // ensembles: the contacts of scenes not yet frozen, compacted colour by colour once
+  // enough scenes have frozen (two lists of C entries, alternately rebuilt), and per-colour
+  // per-workgroup counts for the compaction (kMaxColours x gridDim); nullptr: no compaction
+  uint32_t* act_idx;
+  uint32_t* act_part;
 };
 
 // True if contact k belongs to a scene whose last evaluated residual is below stop.
@@ -257,6 +262,94 @@ __device__ __forceinline__ void keep(const GSArgs<Real>& A, uint32_t k, bool cop
   if (!copy_best) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
+}
+
+// Frozen scenes in the mask (every workgroup computes the same count).
+__device__ __forceinline__ uint32_t count_frozen(uint32_t nscenes, const uint32_t* mask, uint32_t* s_tmp) {
+  uint32_t n = 0;
+  for (uint32_t w = threadIdx.x; w < (nscenes + 31) / 32; w += blockDim.x) n += __popc(mask[w]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = n;
+  __syncthreads();
+  uint32_t t = 0;
+  for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += s_tmp[w];
+  __syncthreads();
+  return t;
+}
+
+// Ensembles: the contacts of the scenes not frozen in `mask`, compacted colour by colour
+// from the list src (nullptr: every contact) with colour offsets soff into dst, with its
+// colour offsets in doff (LDS, ncol + 1 entries). Each workgroup takes one contiguous
+// chunk of each colour and keeps the order of its contacts (spatial within a colour), so
+// the chunks land in order too. Two grid barriers: after the per-chunk counts and after
+// the writes. All threads of the grid take part.
+template <typename Real>
+__device__ void compact_active(const GSArgs<Real>& A, const uint32_t* src, const uint32_t* soff, uint32_t ncol,
+                               const uint32_t* mask, uint32_t* dst, uint32_t* doff) {
+  __shared__ uint32_t s_cnt[kMaxColours];  // this workgroup's chunk count, then the colour totals
+  __shared__ uint32_t s_pre[kMaxColours];  // active contacts of the chunks before this one
+  __shared__ uint32_t s_wave[kGSBlock / 64];
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  auto active = [&](uint32_t j) {
+    const uint32_t sc = A.cscene[src ? src[j] : j];
+    return ((mask[sc >> 5] >> (sc & 31)) & 1u) == 0u;
+  };
+  for (uint32_t c = threadIdx.x; c < kMaxColours; c += kGSBlock) {
+    s_cnt[c] = 0u;
+    s_pre[c] = 0u;
+  }
+  __syncthreads();
+  for (uint32_t col = 0; col < ncol; ++col) {
+    const uint32_t n = soff[col + 1] - soff[col], per = (n + G - 1) / G;
+    const uint32_t lo = soff[col] + min(n, b * per), hi = soff[col] + min(n, (b + 1) * per);
+    for (uint32_t j0 = lo; j0 < hi; j0 += kGSBlock) {
+      const uint32_t j = j0 + threadIdx.x;
+      const unsigned long long bal = __ballot(j < hi && active(j));
+      if (lane == 0 && bal) atomicAdd(s_cnt + col, (uint32_t)__popcll(bal));
+    }
+  }
+  __syncthreads();
+  for (uint32_t col = threadIdx.x; col < ncol; col += kGSBlock) A.act_part[(size_t)col * G + b] = s_cnt[col];
+  grid_barrier(A.bar);
+  for (uint32_t c = threadIdx.x; c < kMaxColours; c += kGSBlock) s_cnt[c] = 0u;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < ncol * G; i += kGSBlock) {
+    const uint32_t v = A.act_part[i], col = i / G;
+    if (v) {
+      atomicAdd(s_cnt + col, v);
+      if (i % G < b) atomicAdd(s_pre + col, v);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    doff[0] = 0u;
+    for (uint32_t col = 0; col < ncol; ++col) doff[col + 1] = doff[col] + s_cnt[col];
+  }
+  __syncthreads();
+  for (uint32_t col = 0; col < ncol; ++col) {
+    const uint32_t n = soff[col + 1] - soff[col], per = (n + G - 1) / G;
+    const uint32_t lo = soff[col] + min(n, b * per), hi = soff[col] + min(n, (b + 1) * per);
+    uint32_t pos = doff[col] + s_pre[col];
+    for (uint32_t j0 = lo; j0 < hi; j0 += kGSBlock) {
+      const uint32_t j = j0 + threadIdx.x;
+      const bool a = j < hi && active(j);
+      const unsigned long long bal = __ballot(a);
+      if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      uint32_t before = 0, total = 0;
+      for (uint32_t w = 0; w < kGSBlock / 64; ++w) {
+        before += (w < wave) ? s_wave[w] : 0u;
+        total += s_wave[w];
+      }
+      if (a) dst[pos + before + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = src ? src[j] : j;
+      pos += total;
+      __syncthreads();
+    }
+  }
+  grid_barrier(A.bar);
 }
 
 // A contact's record and impulse, loaded ahead of its colour pass. Only the thread that
@@ -407,7 +500,11 @@ __device__ __forceinline__ void scene_max(unsigned long long* acc, uint32_t scen
 // launch, base those of earlier phases).
 template <int D, typename Real>
 __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize, uint32_t ev = 0,
-                                double stop = 0.0, uint32_t base = 0, uint32_t it = 0) {
+                                double stop = 0.0, uint32_t base = 0, uint32_t it = 0,
+                                const uint32_t* lst = nullptr, uint32_t nlst = 0) {
+  // ensembles after a compaction: only the contacts of scenes not frozen are evaluated (a
+  // frozen scene met the tolerance and is no longer updated; its buffer entry stays 0)
+  const uint32_t nk = lst ? nlst : A.C;
   __shared__ double s_red[kGSBlock / 64];
   double m = 0.0;
   if (A.cscene) {
@@ -422,10 +519,11 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
       __shared__ unsigned long long s_acc[kSceneLdsMax];
       for (uint32_t sc = threadIdx.x; sc < A.nscenes; sc += kGSBlock) s_acc[sc] = 0ull;
       __syncthreads();
-      const uint32_t per = (A.C + gridDim.x - 1) / gridDim.x;
-      const uint32_t lo = min(A.C, blockIdx.x * per), hi = min(A.C, lo + per);
+      const uint32_t per = (nk + gridDim.x - 1) / gridDim.x;
+      const uint32_t lo = min(nk, blockIdx.x * per), hi = min(nk, lo + per);
 #pragma unroll 4
-      for (uint32_t k = lo + threadIdx.x; k < hi; k += kGSBlock) {
+      for (uint32_t j = lo + threadIdx.x; j < hi; j += kGSBlock) {
+        const uint32_t k = lst ? lst[j] : j;
         const double e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
         m = fmax(m, e);
         if (e > 0.0) atomicMax(s_acc + A.cscene[k], (unsigned long long)__double_as_longlong(e));
@@ -434,9 +532,10 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
       for (uint32_t sc = threadIdx.x; sc < A.nscenes; sc += kGSBlock)
         if (s_acc[sc] != 0ull) atomicMax(acc + sc, s_acc[sc]);
     } else {
-      const uint32_t nk = (A.C + gsize - 1) / gsize * gsize;  // whole waves take part in scene_max
-      for (uint32_t k = gtid; k < nk; k += gsize) {
-        const bool valid = k < A.C;
+      const uint32_t nkw = (nk + gsize - 1) / gsize * gsize;  // whole waves take part in scene_max
+      for (uint32_t j = gtid; j < nkw; j += gsize) {
+        const bool valid = j < nk;
+        const uint32_t k = valid ? (lst ? lst[j] : j) : 0u;
         double e = 0.0;
         if (valid) e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
         m = fmax(m, e);
@@ -519,6 +618,9 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   const uint32_t gtid = blockIdx.x * kGSBlock + threadIdx.x;
   __shared__ uint32_t off[kMaxColours + 1];
   __shared__ uint32_t s_frozen[kSceneMaskBits / 32];
+  __shared__ uint32_t s_aoff[2][kMaxColours + 1];  // colour offsets of the compacted lists
+  __shared__ uint32_t s_tmp[kGSBlock / 64];
+  __shared__ double s_fm[kGSBlock / 64];
   const uint32_t ncol = A.ctr->ncolours;
   for (uint32_t c = threadIdx.x; c <= ncol; c += kGSBlock) off[c] = A.coloff[c];
   __syncthreads();
@@ -540,7 +642,18 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   }
   uint32_t ev = 0;  // evaluations so far; the last one's scene residuals are in buffer (ev - 1) & 1
   double err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, 0);
-  if (A.cscene && A.nscenes <= kSceneMaskBits) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
+  const bool masked = A.cscene && A.nscenes <= kSceneMaskBits;
+  if (masked) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
+  // Ensembles: once enough scenes have frozen, the sweeps and evaluations walk a compacted
+  // list of the other scenes' contacts (lst with colour offsets o) instead of skipping the
+  // frozen ones one by one. A compacted-away scene is no longer evaluated (its residual
+  // reads 0 and it stays frozen); frozen_max keeps the largest residual such scenes had,
+  // so that the reported residual is the maximum over all contacts still.
+  const bool compact = masked && A.act_idx;
+  const uint32_t* lst = nullptr;
+  const uint32_t* o = off;
+  uint32_t nlst = A.C, sel = 0, nfrozen = 0;
+  double frozen_max = 0.0;
   double best = err;
   // Best iterate (So-bogus returns it when the sweep limit is reached): after an
   // evaluation that lowers the best residual, the next sweep stores every contact's
@@ -562,18 +675,21 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       // the first contact of each colour pass is loaded before the barrier that opens it,
       // so that only the body gathers wait for the barrier
       Loaded<D, Real> next;
-      if (tail > 0 && off[0] + gtid < off[1]) load_contact<D, Real>(A, off[0] + gtid, next);
+      if (tail > 0 && o[0] + gtid < o[1]) load_contact<D, Real>(A, lst ? lst[o[0] + gtid] : o[0] + gtid, next);
       for (uint32_t col = 0; col < tail; ++col) {
-        uint32_t k = off[col] + gtid;
-        if (k < off[col + 1]) {
+        uint32_t j = o[col] + gtid;
+        if (j < o[col + 1]) {
+          uint32_t k = lst ? lst[j] : j;
           if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
           else update<D, Real>(A, k, next, false, copy);
-          for (k += gsize; k < off[col + 1]; k += gsize) {
+          for (j += gsize; j < o[col + 1]; j += gsize) {
+            k = lst ? lst[j] : j;
             if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
             else process<D, Real>(A, k, false, copy);
           }
         }
-        if (col + 1 < tail && off[col + 1] + gtid < off[col + 2]) load_contact<D, Real>(A, off[col + 1] + gtid, next);
+        if (col + 1 < tail && o[col + 1] + gtid < o[col + 2])
+          load_contact<D, Real>(A, lst ? lst[o[col + 1] + gtid] : o[col + 1] + gtid, next);
         grid_barrier(A.bar);
       }
       if (tail < ncol) {
@@ -582,7 +698,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         // instead of a grid barrier per colour; the sweep order is unchanged
         if (blockIdx.x == 0)
           for (uint32_t col = tail; col < ncol; ++col) {
-            for (uint32_t k = off[col] + threadIdx.x; k < off[col + 1]; k += kGSBlock) {
+            for (uint32_t j = o[col] + threadIdx.x; j < o[col + 1]; j += kGSBlock) {
+              const uint32_t k = lst ? lst[j] : j;
               if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
               else process<D, Real>(A, k, false, copy);
             }
@@ -591,8 +708,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         grid_barrier(A.bar);
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
-        err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, it);
-        if (A.cscene && A.nscenes <= kSceneMaskBits) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
+        err = fmax(eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, it, lst, nlst), frozen_max);
+        if (masked) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
         if (err < best) {
           best = err;
           copy_pending = track_best;
@@ -603,15 +720,43 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
           ref = best;
           ref_it = it;
         }
+        if (compact) {
+          // compact when the frozen scenes grew by an eighth of those that were left
+          const uint32_t nf = count_frozen(A.nscenes, s_frozen, s_tmp);
+          if (nf > nfrozen && nf - nfrozen >= max(1u, (A.nscenes - nfrozen) / 8)) {
+            // the residuals of the scenes compacted away now (0 for those compacted earlier)
+            double fm = 0.0;
+            for (uint32_t sc = threadIdx.x; sc < A.nscenes; sc += kGSBlock)
+              if ((s_frozen[sc >> 5] >> (sc & 31)) & 1u)
+                fm = fmax(fm, __longlong_as_double((long long)__hip_atomic_load(
+                                  A.scene_acc + (size_t)((ev - 1) & 1) * A.nscenes + sc, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT)));
+            fm = wave_reduce_max_f64(fm);
+            if ((threadIdx.x & 63) == 0) s_fm[threadIdx.x >> 6] = fm;
+            __syncthreads();
+            for (uint32_t w = 0; w < kGSBlock / 64; ++w) frozen_max = fmax(frozen_max, s_fm[w]);
+            uint32_t* dst = A.act_idx + (size_t)sel * A.C;
+            compact_active(A, lst, o, ncol, s_frozen, dst, s_aoff[sel]);
+            lst = dst;
+            o = s_aoff[sel];
+            nlst = o[ncol];
+            sel ^= 1u;
+            nfrozen = nf;
+          }
+        }
       }
     }
     if (it > max_iters && track_best && !copy_pending) {
       // not converged and the current iterate is not the best: return the best iterate
-      // and its velocities (So-bogus keeps x_best)
+      // and its velocities (So-bogus keeps x_best). Ensembles: a frozen scene met the
+      // tolerance and keeps its impulses (each scene is a solve of its own).
       for (uint32_t col = 0; col < ncol; ++col)
-        for (uint32_t k = off[col] + gtid; k < off[col + 1]; k += gsize)
+        for (uint32_t j = o[col] + gtid; j < o[col + 1]; j += gsize) {
+          const uint32_t k = lst ? lst[j] : j;
+          if (frozen_fast(A, k, (ev - 1) & 1, stop, s_frozen)) continue;
 #pragma unroll
           for (int d = 0; d < 3; ++d) A.c.r[3 * (size_t)k + d] = A.rbest[3 * (size_t)k + d];
+        }
       grid_barrier(A.bar);
       apply_impulses<D, Real>(A, off, ncol, gtid, gsize, true);
     }
@@ -1155,6 +1300,10 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
       A.scene_acc = c->d.scene_acc;
       A.scene_sweeps = c->d.scene_sweeps;
       A.nscenes = c->nscenes;
+      if (c->ens_compact) {
+        A.act_idx = c->d.act_idx;
+        A.act_part = c->d.act_part;
+      }
     }
     e = coop_gs<D, float>(c, A, c->gs_blocks, col_off);
     if (e != hipSuccess) return e;
@@ -1174,6 +1323,10 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
     B.scene_acc = c->d.scene_acc;
     B.scene_sweeps = c->d.scene_sweeps;
     B.nscenes = c->nscenes;
+    if (c->ens_compact) {
+      B.act_idx = c->d.act_idx;
+      B.act_part = c->d.act_part;
+    }
   }
   e = coop_gs<D, double>(c, B, c->gs_blocks, col_off);
   if (e != hipSuccess) return e;
