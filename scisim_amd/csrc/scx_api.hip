@@ -577,6 +577,9 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     // SCX_ENSEMBLE_COMPACT=0: ensembles skip frozen contacts one by one instead of sweeping
     // compacted lists (the same iterates bit for bit; for A/B measurements and tests)
     if (const char* env = getenv("SCX_ENSEMBLE_COMPACT")) c->ens_compact = atoi(env) != 0;
+    // SCX_COHERENT_SWEEPS=0: the colour passes use ordinary body accesses and full grid
+    // barriers (the same iterates bit for bit; for A/B measurements and tests)
+    if (const char* env = getenv("SCX_COHERENT_SWEEPS")) c->coherent_sweeps = atoi(env) != 0;
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
         (e = dalloc(&d.act_part, (size_t)kMaxColours * std::max(c->gs_blocks, 1))) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));

@@ -252,6 +252,7 @@ struct scx_ctx {
   int colour_blocks = 0;
   int force_blocks = 0;  // > 0: persistent kernels use this many workgroups (capped), for tests
   bool ens_compact = true;  // ensembles: sweep compacted lists of the unfrozen scenes' contacts
+  bool coherent_sweeps = true;  // colour passes with coherent body accesses and light barriers
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
   // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
   // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need

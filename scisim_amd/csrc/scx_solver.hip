@@ -175,6 +175,76 @@ __device__ __forceinline__ T fb_err(T mu, const T* r, const T* u) {
 template <int D>
 constexpr int kBodyVecs = (D == 3) ? 2 : 1;
 
+// Coherent accesses to the body records for the colour passes between grid_sync_coherent
+// barriers: buffer loads and stores with the agent-scope coherence bit (SC1), which go to
+// the memory side past the per-XCD L2s, so that a record another workgroup updated in the
+// previous pass is seen without an L2 invalidate. The records are addressed by their
+// 16-byte (float) or 32-byte (double) vectors; the buffer covers nbodies records (the host
+// keeps it below 2 GiB).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCoherentAux = 16;  // cache policy bits: SC1
+
+template <typename Real>
+struct BodyPort;
+
+template <>
+struct BodyPort<float> {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ float4 ld(size_t v) const {
+    const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)v * 16u, 0, kCoherentAux);
+    return make_float4(__uint_as_float(x[0]), __uint_as_float(x[1]), __uint_as_float(x[2]), __uint_as_float(x[3]));
+  }
+  __device__ __forceinline__ void st(size_t v, const float4& f) const {
+    const u32x4 x = {__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w)};
+    __builtin_amdgcn_raw_buffer_store_b128(x, rs, (uint32_t)v * 16u, 0, kCoherentAux);
+  }
+};
+
+template <>
+struct BodyPort<double> {
+  __amdgpu_buffer_rsrc_t rs;
+  static __device__ __forceinline__ double join(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+  __device__ __forceinline__ double4 ld(size_t v) const {
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)v * 32u, 0, kCoherentAux);
+    const u32x4 b = __builtin_amdgcn_raw_buffer_load_b128(rs, (uint32_t)v * 32u + 16u, 0, kCoherentAux);
+    return make_double4(join(a[0], a[1]), join(a[2], a[3]), join(b[0], b[1]), join(b[2], b[3]));
+  }
+  __device__ __forceinline__ void st(size_t v, const double4& d) const {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(d.x);
+    const unsigned long long y = (unsigned long long)__double_as_longlong(d.y);
+    const unsigned long long z = (unsigned long long)__double_as_longlong(d.z);
+    const unsigned long long w = (unsigned long long)__double_as_longlong(d.w);
+    const u32x4 a = {(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32)};
+    const u32x4 b = {(uint32_t)z, (uint32_t)(z >> 32), (uint32_t)w, (uint32_t)(w >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(a, rs, (uint32_t)v * 32u, 0, kCoherentAux);
+    __builtin_amdgcn_raw_buffer_store_b128(b, rs, (uint32_t)v * 32u + 16u, 0, kCoherentAux);
+  }
+};
+
+template <int D, typename Real>
+__device__ __forceinline__ BodyPort<Real> body_port(Real* body, uint32_t nbodies) {
+  BodyPort<Real> P;
+  const uint32_t bytes = nbodies * (uint32_t)(kBodyVecs<D> * 4 * sizeof(Real));
+  P.rs = __builtin_amdgcn_make_buffer_rsrc(body, (short)0, (int)bytes, 0x00020000);
+  return P;
+}
+
+// Coherent (SC1) load and store of one impulse component: with coherent colour passes the
+// impulses are read by the residual evaluation in another thread's mapping, so they take
+// the same path to memory as the body records.
+template <bool Coh, typename Real>
+__device__ __forceinline__ Real ld_imp(const Real* p) {
+  return Coh ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+}
+template <bool Coh, typename Real>
+__device__ __forceinline__ void st_imp(Real* p, Real x) {
+  if (Coh) __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = x;
+}
+
+
 template <typename Real>
 struct GSArgs {
   ContactSoA<Real> c;
@@ -257,11 +327,11 @@ __device__ __forceinline__ bool frozen_fast(const GSArgs<Real>& A, uint32_t k, u
 }
 
 // A frozen contact keeps its impulse; the lazy best-iterate copy still stores it.
-template <typename Real>
+template <bool Coh = false, typename Real>
 __device__ __forceinline__ void keep(const GSArgs<Real>& A, uint32_t k, bool copy_best) {
   if (!copy_best) return;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = A.c.r[3 * (size_t)k + d];
+  for (int d = 0; d < 3; ++d) A.rbest[3 * (size_t)k + d] = ld_imp<Coh>(A.c.r + 3 * (size_t)k + d);
 }
 
 // Frozen scenes in the mask (every workgroup computes the same count).
@@ -361,32 +431,37 @@ struct Loaded {
   Real r[3];
 };
 
-template <int D, typename Real>
+template <int D, typename Real, bool Coh = false>
 __device__ __forceinline__ void load_contact(const GSArgs<Real>& A, uint32_t k, Loaded<D, Real>& L) {
   load_rec<D, Real>(A.c, k, L.R);
   const Real* rp = A.c.r + 3 * (size_t)k;
-  L.r[0] = rp[0];
-  L.r[1] = rp[1];
-  L.r[2] = rp[2];
+  L.r[0] = ld_imp<Coh>(rp);
+  L.r[1] = ld_imp<Coh>(rp + 1);
+  L.r[2] = ld_imp<Coh>(rp + 2);
 }
 
 // One contact of a colour pass from its loaded record. copy_best: first store the
 // contact's impulse as it was before this sweep into rbest (the previous sweep's iterate
 // was the best one so far).
-template <int D, typename Real>
+template <int D, typename Real, bool Coh = false>
 __device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const Loaded<D, Real>& L, bool init,
-                                       bool copy_best) {
+                                       bool copy_best, const BodyPort<Real>& P = BodyPort<Real>{}) {
   using V = typename Vec4<Real>::type;
   const Rec<D, Real>& R = L.R;
   const bool hasb = R.b != kNoBody;
   V* bv = reinterpret_cast<V*>(A.body);
-  V* bw = bv + 1;  // 3D: the angular half of the same record
   const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
-  V va = bv[ia], wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
-  if (D == 3) wa = bw[ia];
+  // body vector i (3D: i + 1 is the angular half of the same record)
+  auto ld = [&](size_t i) { return Coh ? P.ld(i) : bv[i]; };
+  auto st = [&](size_t i, const V& x) {
+    if (Coh) P.st(i, x);
+    else bv[i] = x;
+  };
+  V va = ld(ia), wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
+  if (D == 3) wa = ld(ia + 1);
   if (hasb) {
-    vb = bv[ib];
-    if (D == 3) wb = bw[ib];
+    vb = ld(ib);
+    if (D == 3) wb = ld(ib + 1);
   }
   Real* rp = A.c.r + 3 * (size_t)k;
   const Real r[3] = {L.r[0], L.r[1], L.r[2]};
@@ -408,17 +483,17 @@ __device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const 
     dr[0] = rn[0] - r[0];
     dr[1] = rn[1] - r[1];
     dr[2] = rn[2] - r[2];
-    rp[0] = rn[0];
-    rp[1] = rn[1];
-    if (D == 3) rp[2] = rn[2];
+    st_imp<Coh>(rp, rn[0]);
+    st_imp<Coh>(rp + 1, rn[1]);
+    if (D == 3) st_imp<Coh>(rp + 2, rn[2]);
   }
-  Real P[3];
+  Real Pi[3];  // the impulse increment in world coordinates
 #pragma unroll
-  for (int d = 0; d < 3; ++d) P[d] = R.n[d] * dr[0] + R.t[d] * dr[D - 1] + ((D == 3) ? R.s[d] * dr[1] : (Real)0);
-  va.x += va.w * P[0];
-  va.y += va.w * P[1];
-  va.z += va.w * P[2];
-  bv[ia] = va;
+  for (int d = 0; d < 3; ++d) Pi[d] = R.n[d] * dr[0] + R.t[d] * dr[D - 1] + ((D == 3) ? R.s[d] * dr[1] : (Real)0);
+  va.x += va.w * Pi[0];
+  va.y += va.w * Pi[1];
+  va.z += va.w * Pi[2];
+  st(ia, va);
   Real tau[3];
   if (D == 3) {
 #pragma unroll
@@ -427,48 +502,49 @@ __device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const 
     wa.x += ka * tau[0];
     wa.y += ka * tau[1];
     wa.z += ka * tau[2];
-    bw[ia] = wa;
+    st(ia + 1, wa);
   }
   if (hasb) {
-    vb.x -= vb.w * P[0];
-    vb.y -= vb.w * P[1];
-    vb.z -= vb.w * P[2];
-    bv[ib] = vb;
+    vb.x -= vb.w * Pi[0];
+    vb.y -= vb.w * Pi[1];
+    vb.z -= vb.w * Pi[2];
+    st(ib, vb);
     if (D == 3) {
       const Real kb = R.lb * wb.w;
       wb.x -= kb * tau[0];
       wb.y -= kb * tau[1];
       wb.z -= kb * tau[2];
-      bw[ib] = wb;
+      st(ib + 1, wb);
     }
   }
 }
 
 // One contact of a colour pass (record loaded here).
-template <int D, typename Real>
-__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init, bool copy_best = false) {
+template <int D, typename Real, bool Coh = false>
+__device__ __forceinline__ void process(const GSArgs<Real>& A, uint32_t k, bool init, bool copy_best = false,
+                                        const BodyPort<Real>& P = BodyPort<Real>{}) {
   Loaded<D, Real> L;
-  load_contact<D, Real>(A, k, L);
-  update<D, Real>(A, k, L, init, copy_best);
+  load_contact<D, Real, Coh>(A, k, L);
+  update<D, Real, Coh>(A, k, L, init, copy_best, P);
 }
 
-template <int D, typename Real, typename T = double>
-__device__ __forceinline__ T contact_err(const GSArgs<Real>& A, uint32_t k) {
+template <int D, typename Real, typename T = double, bool Coh = false>
+__device__ __forceinline__ T contact_err(const GSArgs<Real>& A, uint32_t k, const BodyPort<Real>& P = BodyPort<Real>{}) {
   using V = typename Vec4<Real>::type;
   Rec<D, Real> R;
   load_rec<D, Real>(A.c, k, R);
   const V* bv = reinterpret_cast<const V*>(A.body);
-  const V* bw = bv + 1;
   const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
+  auto ld = [&](size_t i) { return Coh ? P.ld(i) : bv[i]; };
   const V z = {0, 0, 0, 0};
-  const V va = bv[ia];
-  const V wa = (D == 3) ? bw[ia] : z;
-  const V vb = (R.b != kNoBody) ? bv[ib] : z;
-  const V wb = (D == 3 && R.b != kNoBody) ? bw[ib] : z;
+  const V va = ld(ia);
+  const V wa = (D == 3) ? ld(ia + 1) : z;
+  const V vb = (R.b != kNoBody) ? ld(ib) : z;
+  const V wb = (D == 3 && R.b != kNoBody) ? ld(ib + 1) : z;
   T u[3];
   rel_vel<D, T, Real>(R, va, wa, vb, wb, u);
   const Real* rp = A.c.r + 3 * (size_t)k;
-  const T r[3] = {(T)rp[0], (T)rp[1], (T)rp[2]};
+  const T r[3] = {(T)ld_imp<Coh>(rp), (T)ld_imp<Coh>(rp + 1), (T)ld_imp<Coh>(rp + 2)};
   return fb_err<D, T>((T)R.mu, r, u);
 }
 
@@ -498,10 +574,13 @@ __device__ __forceinline__ void scene_max(unsigned long long* acc, uint32_t scen
 // also the per-scene max into buffer ev & 1 (the other buffer is cleared for the next
 // evaluation), and the sweep at which each scene meets `stop` (it sweeps done in this
 // launch, base those of earlier phases).
-template <int D, typename Real>
+// Coh: after coherent colour passes; the body records and impulses are read coherently and
+// the reduction's barrier is grid_sync_coherent.
+template <int D, typename Real, bool Coh = false>
 __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize, uint32_t ev = 0,
                                 double stop = 0.0, uint32_t base = 0, uint32_t it = 0,
-                                const uint32_t* lst = nullptr, uint32_t nlst = 0) {
+                                const uint32_t* lst = nullptr, uint32_t nlst = 0,
+                                const BodyPort<Real>& P = BodyPort<Real>{}) {
   // ensembles after a compaction: only the contacts of scenes not frozen are evaluated (a
   // frozen scene met the tolerance and is no longer updated; its buffer entry stays 0)
   const uint32_t nk = lst ? nlst : A.C;
@@ -524,7 +603,7 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
 #pragma unroll 4
       for (uint32_t j = lo + threadIdx.x; j < hi; j += kGSBlock) {
         const uint32_t k = lst ? lst[j] : j;
-        const double e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
+        const double e = (A.phase == 1) ? (double)contact_err<D, Real, float, Coh>(A, k, P) : contact_err<D, Real, double, Coh>(A, k, P);
         m = fmax(m, e);
         if (e > 0.0) atomicMax(s_acc + A.cscene[k], (unsigned long long)__double_as_longlong(e));
       }
@@ -537,7 +616,7 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
         const bool valid = j < nk;
         const uint32_t k = valid ? (lst ? lst[j] : j) : 0u;
         double e = 0.0;
-        if (valid) e = (A.phase == 1) ? (double)contact_err<D, Real, float>(A, k) : contact_err<D, Real>(A, k);
+        if (valid) e = (A.phase == 1) ? (double)contact_err<D, Real, float, Coh>(A, k, P) : contact_err<D, Real, double, Coh>(A, k, P);
         m = fmax(m, e);
         scene_max(acc, valid ? A.cscene[k] : 0u, e, valid);
       }
@@ -545,10 +624,10 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
   } else if (A.phase == 1) {  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
     // read-only loop: unrolled so that the loads of several contacts are in flight together
 #pragma unroll 4
-    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float>(A, k));
+    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float, Coh>(A, k, P));
   } else {
 #pragma unroll 4
-    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real>(A, k));
+    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real, double, Coh>(A, k, P));
   }
   m = wave_reduce_max_f64(m);
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
@@ -556,9 +635,10 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
   if (threadIdx.x == 0) {
     double b = s_red[0];
     for (int w = 1; w < kGSBlock / 64; ++w) b = fmax(b, s_red[w]);
-    A.partial[blockIdx.x] = b;
+    __hip_atomic_store(A.partial + blockIdx.x, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  grid_barrier(A.bar);
+  if (Coh) grid_sync_coherent(A.bar);
+  else grid_barrier(A.bar);
   if (A.cscene) {
     unsigned long long* acc = A.scene_acc + (size_t)(ev & 1) * A.nscenes;
     for (uint32_t sc = gtid; sc < A.nscenes; sc += gsize) {
@@ -572,7 +652,8 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
     }
   }
   double e = 0.0;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kGSBlock) e = fmax(e, A.partial[b]);
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kGSBlock)
+    e = fmax(e, __hip_atomic_load(A.partial + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   e = wave_reduce_max_f64(e);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = e;
@@ -612,7 +693,12 @@ __device__ void apply_impulses(const GSArgs<Real>& A, const uint32_t* off, uint3
   }
 }
 
-template <int D, typename Real>
+// Coh: the colour passes and the residual evaluations access the body records and the
+// impulses coherently (BodyPort, ld_imp/st_imp) and are separated by grid_sync_coherent;
+// the small colours that workgroup 0 sweeps alone go through its L2 between an agent-scope
+// acquire and release; the initial application of the impulses, the compaction and the
+// best-iterate return keep full barriers and ordinary accesses.
+template <int D, typename Real, bool Coh>
 __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   const uint32_t gsize = gridDim.x * kGSBlock;
   const uint32_t gtid = blockIdx.x * kGSBlock + threadIdx.x;
@@ -641,7 +727,8 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
     grid_barrier(A.bar);
   }
   uint32_t ev = 0;  // evaluations so far; the last one's scene residuals are in buffer (ev - 1) & 1
-  double err = eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, 0);
+  const BodyPort<Real> P = body_port<D, Real>(A.body, A.nbodies);
+  double err = eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, 0, nullptr, 0, P);
   const bool masked = A.cscene && A.nscenes <= kSceneMaskBits;
   if (masked) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
   // Ensembles: once enough scenes have frozen, the sweeps and evaluations walk a compacted
@@ -672,31 +759,47 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       copy_pending = false;
       const uint32_t eb = (ev - 1) & 1;
       const uint32_t tail = min(A.tail_col, ncol);
-      // the first contact of each colour pass is loaded before the barrier that opens it,
-      // so that only the body gathers wait for the barrier
+      const bool eval_now = A.eval_every > 0 && it % A.eval_every == 0;
+      // the first contact of each colour pass is loaded before the barrier that opens it
+      // completes (between the arrival and the wait of a coherent barrier), so that only the
+      // body gathers wait for the barrier
       Loaded<D, Real> next;
-      if (tail > 0 && o[0] + gtid < o[1]) load_contact<D, Real>(A, lst ? lst[o[0] + gtid] : o[0] + gtid, next);
+      if (tail > 0 && o[0] + gtid < o[1]) load_contact<D, Real, Coh>(A, lst ? lst[o[0] + gtid] : o[0] + gtid, next);
       for (uint32_t col = 0; col < tail; ++col) {
         uint32_t j = o[col] + gtid;
         if (j < o[col + 1]) {
           uint32_t k = lst ? lst[j] : j;
-          if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
-          else update<D, Real>(A, k, next, false, copy);
+          if (frozen_fast(A, k, eb, stop, s_frozen)) keep<Coh>(A, k, copy);
+          else update<D, Real, Coh>(A, k, next, false, copy, P);
           for (j += gsize; j < o[col + 1]; j += gsize) {
             k = lst ? lst[j] : j;
-            if (frozen_fast(A, k, eb, stop, s_frozen)) keep(A, k, copy);
-            else process<D, Real>(A, k, false, copy);
+            if (frozen_fast(A, k, eb, stop, s_frozen)) keep<Coh>(A, k, copy);
+            else process<D, Real, Coh>(A, k, false, copy, P);
           }
         }
-        if (col + 1 < tail && o[col + 1] + gtid < o[col + 2])
-          load_contact<D, Real>(A, lst ? lst[o[col + 1] + gtid] : o[col + 1] + gtid, next);
-        grid_barrier(A.bar);
+        const bool pre = col + 1 < tail && o[col + 1] + gtid < o[col + 2];
+        if (Coh) {
+          const GridTicket t = grid_arrive_coherent(A.bar);
+          if (pre) load_contact<D, Real, Coh>(A, lst ? lst[o[col + 1] + gtid] : o[col + 1] + gtid, next);
+          grid_wait_coherent(A.bar, t);
+        } else {
+          if (pre) load_contact<D, Real>(A, lst ? lst[o[col + 1] + gtid] : o[col + 1] + gtid, next);
+          grid_barrier(A.bar);
+        }
       }
       if (tail < ncol) {
         // the last, small colours (typically the contacts of a few bodies of high degree):
         // one workgroup sweeps them in order with workgroup barriers, then one grid barrier,
         // instead of a grid barrier per colour; the sweep order is unchanged
-        if (blockIdx.x == 0)
+        // With coherent passes the workgroup works through its L2 with ordinary accesses:
+        // an agent-scope acquire first (drops stale lines of records other workgroups
+        // updated), an agent-scope release at the end (writes its updates back to memory,
+        // where the coherent loads of the next pass and of the evaluation read them).
+        if (blockIdx.x == 0) {
+          if (Coh) {
+            if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __syncthreads();
+          }
           for (uint32_t col = tail; col < ncol; ++col) {
             for (uint32_t j = o[col] + threadIdx.x; j < o[col + 1]; j += kGSBlock) {
               const uint32_t k = lst ? lst[j] : j;
@@ -705,10 +808,17 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
             }
             __syncthreads();
           }
-        grid_barrier(A.bar);
+          if (Coh) {
+            wait_stores();
+            __syncthreads();
+            if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          }
+        }
+        if (Coh) grid_sync_coherent(A.bar);
+        else grid_barrier(A.bar);
       }
-      if (A.eval_every > 0 && it % A.eval_every == 0) {
-        err = fmax(eval_residual<D, Real>(A, gtid, gsize, ev++, stop, base, it, lst, nlst), frozen_max);
+      if (eval_now) {
+        err = fmax(eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, it, lst, nlst, P), frozen_max);
         if (masked) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
         if (err < best) {
           best = err;
@@ -749,7 +859,9 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
     if (it > max_iters && track_best && !copy_pending) {
       // not converged and the current iterate is not the best: return the best iterate
       // and its velocities (So-bogus keeps x_best). Ensembles: a frozen scene met the
-      // tolerance and keeps its impulses (each scene is a solve of its own).
+      // tolerance and keeps its impulses (each scene is a solve of its own). A full barrier
+      // first: the copies and impulses are read here in another thread's mapping.
+      grid_barrier(A.bar);
       for (uint32_t col = 0; col < ncol; ++col)
         for (uint32_t j = o[col] + gtid; j < o[col + 1]; j += gsize) {
           const uint32_t k = lst ? lst[j] : j;
@@ -1216,7 +1328,11 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
     if (col_off[b + 1] > col_off[b]) widths[nw++] = col_off[b + 1] - col_off[b];
   if (tail < ncol) widths[nw++] = 1;
   widths[nw++] = A.C;
-  const int model = c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks);
+  // coherent colour passes unless switched off or the body records exceed the 32-bit
+  // byte offsets of the buffer accesses
+  const bool coh = c->coherent_sweeps && (uint64_t)A.nbodies * kBodyVecs<D> * 4 * sizeof(Real) < (1ull << 31);
+  const int model =
+      c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks, coh);
   const size_t lds = LdsLayout<D, Real>::bytes(A.C);
   const bool lds_path = std::min(max_blocks, model) <= 1 && c->force_blocks <= 0 && lds <= kGSLdsMax && A.lmap &&
                         !A.cscene;
@@ -1238,13 +1354,14 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
   void* args[] = {&A};
   hipError_t e = tune ? hipEventRecord(c->tuner.pend[phase].ev[0], c->stream) : hipSuccess;
   if (e != hipSuccess) return e;
+  const void* fn = coh ? (const void*)k_gs<D, Real, true> : (const void*)k_gs<D, Real, false>;
   bool launched = false;
-  e = launch_cooperative(c, (const void*)k_gs<D, Real>, blocks, kGSBlock, args, 0, c->stream, &launched);
+  e = launch_cooperative(c, fn, blocks, kGSBlock, args, 0, c->stream, &launched);
   if (e == hipSuccess && !launched) {
     // one workgroup (chosen, or after a refused cooperative launch: see
     // scx_ctx::coop_unavailable) sweeps every colour itself, so there is no tail split
     A.tail_col = (uint32_t)kMaxColours;
-    e = hipLaunchKernel((const void*)k_gs<D, Real>, dim3(1), dim3(kGSBlock), args, 0, c->stream);
+    e = hipLaunchKernel(fn, dim3(1), dim3(kGSBlock), args, 0, c->stream);
   }
   if (e == hipSuccess && tune) e = hipEventRecord(c->tuner.pend[phase].ev[1], c->stream);
   return e;
@@ -1253,10 +1370,13 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
 template <int D, typename Real>
 int gs_max_blocks(scx_ctx* c) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gs<D, Real>, kGSBlock, 0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  return per_cu * c->num_cus;
+  int per_cu_coh = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gs<D, Real, false>, kGSBlock, 0) !=
+          hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_coh, (const void*)k_gs<D, Real, true>, kGSBlock, 0) !=
+          hipSuccess)
+    per_cu = per_cu_coh = 1;
+  return std::max(1, std::min(per_cu, per_cu_coh)) * c->num_cus;
 }
 
 // Ensembles: the scene of each contact in colour order (the scene of its body a).

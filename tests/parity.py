@@ -62,6 +62,19 @@ def check_with_oracle(dim, state0, contacts, mu, CoR=0.0):
     return err, vout
 
 
+def position_excess(q, q_ref, v_pairs, dt):
+    """max |q - q_ref| over what the velocity differences allow: every map's positions are
+    q0 plus dt times a combination (weights at most 1) of the start-of-step, predictor and
+    solved velocities, so from the same q0 the positions differ by at most
+    dt * max |v - v_ref| over the given (v, v_ref) pairs, plus a few units in the last
+    place of q. A ratio above 1 is a position update that does not follow the velocities."""
+    dq = float(np.abs(np.asarray(q, dtype=np.float64) - np.asarray(q_ref, dtype=np.float64)).max())
+    dv = max(float(np.abs(np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64)).max())
+             for a, b in v_pairs)
+    ulps = 1e-15 * max(1.0, float(np.abs(np.asarray(q_ref)).max()))
+    return dq / (dt * dv + ulps)
+
+
 def rel(x, y, scale=None):
     """max |x - y| over max |y| (or the given scale)."""
     x = np.asarray(x, dtype=np.float64).ravel()

@@ -119,3 +119,49 @@ def test_learned_warm_start_cuts_sweeps_on_a_settled_pile():
     ctx.close()
     _say("samples", X.shape[0], "loss", loss, "sweeps", sweeps)
     assert sum(sweeps["learned"]) * 2 <= sum(sweeps["cold"])
+
+
+def test_learned_prediction_beats_the_transported_impulse_on_held_out_steps():
+    """The learned warm start against the transported cache it starts from, on steps the
+    perceptron never saw: a settled pile of 20,000 spheres yields (feature, converged
+    impulse) samples of 16 transported-cache steps; the first 12 train the model on the
+    device, the last 4 are held out. The device's own mean-squared error of the prediction
+    (scx_warm_model_gradients) on the held-out samples must be lower with the trained model
+    than with the zero network, whose prediction is the transported impulse itself: over
+    all contacts, and by a wider margin on the contacts new in their step (no cached
+    impulse, feature 11 = 0), where the transported guess is zero."""
+    from scisim_amd import scenes, warmstart
+    n = 20000
+    s = scenes.spheres_pile(n=n, seed=6)
+    ctx = scx.Context(3, n, max_planes=8)
+    ctx.set_planes(s["px"], s["pn"])
+    ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s["inertia"])
+    base = dict(dt=0.005, g=[0.0, -9.81, 0.0], mu=0.4, tol=1e-6, integrator=1)
+    settle = scx.step_params(precision=scx.FP32_POLISH, warm_start=scx.WARM_TRANSPORT, max_iters=1000, **base)
+    for _ in range(300):
+        ctx.step(settle)
+    tp = scx.step_params(precision=scx.FP32_POLISH, warm_start=scx.WARM_TRANSPORT, max_iters=50000, **base)
+    Xs, Ts = [], []
+    for _ in range(16):
+        st = ctx.step(tp)
+        assert st["succeeded"], st
+        X, T = ctx.samples()
+        Xs.append(X)
+        Ts.append(T)
+    Xtr, Ttr = np.concatenate(Xs[:12]), np.concatenate(Ts[:12])
+    Xte, Tte = np.concatenate(Xs[12:]), np.concatenate(Ts[12:])
+    new = Xte[:, 11] == 0
+
+    def mse(X, T):
+        return ctx.warm_model_gradients(X, T)[1]
+
+    ctx.set_warm_model(warmstart.init_params(seed=2))
+    base_all, base_new = mse(Xte, Tte), mse(Xte[new], Tte[new])
+    ctx.train_warm_model(Xtr, Ttr, epochs=12, batch=2048, lr=1e-3, seed=3)
+    got_all, got_new = mse(Xte, Tte), mse(Xte[new], Tte[new])
+    ctx.close()
+    _say("train", Xtr.shape[0], "held out", Xte.shape[0], "new", int(new.sum()), "mse transported / learned: all",
+         base_all, got_all, "new", base_new, got_new)
+    assert new.sum() >= 100
+    assert got_all < base_all
+    assert got_new < 0.8 * base_new

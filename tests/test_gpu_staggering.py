@@ -19,7 +19,7 @@ import pytest
 
 import oracle
 import scisim_amd as scx
-from parity import check_with_oracle, rel, same_active_set
+from parity import check_with_oracle, position_excess, rel, same_active_set
 from scisim_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -96,7 +96,8 @@ def _lockstep(ctx, sim, prm, warm, nsteps, tag, stag_kw):
             continue
         q, v, w = ctx.get_state()
         amax = max(1e-300, np.abs(oc["alpha"]).max())
-        e = {"v2": rel(ctx.v2()[0], sim.v2()), "v1": rel(v, sim.v), "q1": rel(q, sim.q, 1.0)}
+        e = {"v2": rel(ctx.v2()[0], sim.v2()), "v1": rel(v, sim.v),
+             "q1": position_excess(q, sim.q, [(v, sim.v), (ctx.v2()[0], sim.v2())], prm["dt"])}
         # impulses are compared where both solves converged: at the sweep limit the best
         # iterates of a statically indeterminate contact set (a body resting on several
         # contacts) may differ along the impulses that leave every velocity unchanged
@@ -116,7 +117,7 @@ def _lockstep(ctx, sim, prm, warm, nsteps, tag, stag_kw):
         if key in worst:
             assert worst[key] < 1e-9, (key, worst[key])
     if "q1" in worst:
-        assert worst["q1"] < 1e-12
+        assert worst["q1"] <= 1.0
     return worst
 
 
