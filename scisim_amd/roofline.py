@@ -90,7 +90,7 @@ def kernel_trace_summary(trace_csv, kernel_prefix=None, last=0):
     total = sum(s[1] for s in stats.values()) or 1.0
     out = ["| kernel | calls | avg us | min us | max us | total us | % |", "|---|---|---|---|---|---|---|"]
     for name, (n, tot, mn, mx) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
-        short = name.split("(")[0].replace("void ", "").replace("scx::", "")
+        short = _short(name)
         out.append("| `%s` | %d | %.1f | %.1f | %.1f | %.1f | %.1f |" % (short, n, tot / n, mn, mx, tot, 100.0 * tot / total))
     if kernel_prefix and last > 0:
         sel = [us for _, name, us in rows
@@ -102,11 +102,68 @@ def kernel_trace_summary(trace_csv, kernel_prefix=None, last=0):
     return "\n".join(out)
 
 
+def _short(name):
+    return name.split("(")[0].replace("void ", "").replace("scx::", "")
+
+
+def pmc_values(counter_csv, counter, kernel_prefix, last=0):
+    """Per-dispatch values of one counter from a rocprofv3 --pmc CSV (one row per dispatch
+    and counter: Dispatch_Id, Kernel_Name, Counter_Name, Counter_Value; the byte counters
+    are in KiB), for the dispatches whose kernel name starts with kernel_prefix, in
+    dispatch order (summed over the rows of one dispatch); last > 0 keeps the last ones."""
+    import csv
+    per = {}
+    with open(counter_csv, newline="") as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter or not _short(r.get("Kernel_Name", "")).startswith(kernel_prefix):
+                continue
+            d = int(r["Dispatch_Id"])
+            per[d] = per.get(d, 0.0) + float(r["Counter_Value"])
+    vals = [per[d] for d in sorted(per)]
+    return vals[-last:] if last > 0 else vals
+
+
+def hbm_traffic(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv, copy_bytes, kernel_prefix, last=0,
+                copy_kernel="__amd_rocclr_copyBuffer"):
+    """HBM bytes per launch of a kernel from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for a plain
+    device copy of copy_bytes (read and written once): traffic = fetch * fetch_factor +
+    write * write_factor, averaged over the kernel's (last) dispatches."""
+    cf = pmc_values(copy_fetch_csv, "FETCH_SIZE", copy_kernel)
+    cw = pmc_values(copy_write_csv, "WRITE_SIZE", copy_kernel)
+    kf = pmc_values(fetch_csv, "FETCH_SIZE", kernel_prefix, last)
+    kw = pmc_values(write_csv, "WRITE_SIZE", kernel_prefix, last)
+    if not (cf and cw and kf and kw):
+        raise ValueError("missing dispatches in the counter files")
+    ff = copy_bytes / (1024.0 * sum(cf) / len(cf))
+    fw = copy_bytes / (1024.0 * sum(cw) / len(cw))
+    fetch = 1024.0 * sum(kf) / len(kf) * ff
+    write = 1024.0 * sum(kw) / len(kw) * fw
+    return {"bytes_per_launch": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "fetch_factor": ff,
+            "write_factor": fw, "dispatches": [len(kf), len(kw)], "kernel": kernel_prefix,
+            "method": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by a %d-byte device copy" % copy_bytes}
+
+
 if __name__ == "__main__":
     import argparse
-    ap = argparse.ArgumentParser(description="Kernel statistics of a rocprofv3 --kernel-trace CSV")
-    ap.add_argument("trace_csv")
+    import json
+    ap = argparse.ArgumentParser(description="Kernel statistics of a rocprofv3 --kernel-trace CSV, or the HBM "
+                                             "traffic of a kernel from rocprofv3 --pmc CSVs")
+    ap.add_argument("trace_csv", nargs="?")
     ap.add_argument("--kernel", default=None, help="name prefix of the kernel whose last dispatches are averaged")
     ap.add_argument("--last", type=int, default=0, help="number of last dispatches to average")
+    ap.add_argument("--pmc-fetch", help="FETCH_SIZE counter CSV of the workload")
+    ap.add_argument("--pmc-write", help="WRITE_SIZE counter CSV of the workload")
+    ap.add_argument("--copy-fetch", help="FETCH_SIZE counter CSV of the copy calibration")
+    ap.add_argument("--copy-write", help="WRITE_SIZE counter CSV of the copy calibration")
+    ap.add_argument("--copy-bytes", type=int, default=0, help="bytes of the calibration copy")
+    ap.add_argument("--json", help="write the traffic figures to this file (bench.py --traffic reads it)")
     a = ap.parse_args()
-    print(kernel_trace_summary(a.trace_csv, a.kernel, a.last))
+    if a.pmc_fetch:
+        t = hbm_traffic(a.pmc_fetch, a.pmc_write, a.copy_fetch, a.copy_write, a.copy_bytes, a.kernel, a.last)
+        print(json.dumps(t))
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(t, f)
+    else:
+        print(kernel_trace_summary(a.trace_csv, a.kernel, a.last))

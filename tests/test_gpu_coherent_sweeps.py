@@ -7,7 +7,7 @@ return and compactions still follow full barriers. The arithmetic is the same, s
 iterates must be those of the full-barrier kernel (SCX_COHERENT_SWEEPS=0) bit for bit:
 positions, velocities, impulses, sweep counts and residuals, over several steps of a 2D
 circle packing, a 3D sphere pile and an ensemble, in FP64 and in FP32 sweeps with the FP64
-polish, on one workgroup, on 16 workgroups and on the width the cost model picks, with
+polish, on one workgroup, on 16 and 64 workgroups and on the width the cost model picks, with
 steps that converge and with a sweep limit that returns the best iterate.
 """
 
@@ -93,7 +93,7 @@ def _run(monkeypatch, kind, data, coherent, blocks, precision, steps, limit):
 
 @pytest.mark.parametrize("limit", ["converged", "sweep_limit"])
 @pytest.mark.parametrize("precision", [scx.FP32_POLISH, scx.FP64], ids=["fp32_polish", "fp64"])
-@pytest.mark.parametrize("blocks", [1, 16, 0])
+@pytest.mark.parametrize("blocks", [1, 16, 64, 0])
 @pytest.mark.parametrize("kind", ["circles", "pile", "ensemble"])
 def test_coherent_passes_match_full_barriers_bit_for_bit(monkeypatch, request, kind, blocks, precision, limit):
     data = request.getfixturevalue(kind)

@@ -61,3 +61,34 @@ def test_kernel_trace_summary_averages_the_last_dispatches(tmp_path):
     out = rl.kernel_trace_summary(str(p), "k_gs", 2)
     assert "| `k_gs<3, float>` | 6 | 3.5 | 1.0 | 6.0 | 21.0 |" in out
     assert "Last 2 dispatches of `k_gs*` (the timed steps): mean 5.5 us" in out
+
+
+def test_hbm_traffic_from_counter_csvs(tmp_path):
+    """Per-launch HBM bytes from FETCH_SIZE / WRITE_SIZE runs (KiB per dispatch), each scaled
+    by what the counter reads for a plain copy of known size; the last dispatches only."""
+    from scisim_amd import roofline as rl
+    head = "Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n"
+
+    def write(name, rows):
+        p = tmp_path / name
+        p.write_text(head + "".join('%d,"%s",%s,%s\n' % r for r in rows))
+        return str(p)
+
+    # the copy of 1 MiB reads 512 KiB by FETCH_SIZE (factor 2) and writes 1024 KiB (factor 1)
+    cf = write("cf.csv", [(1, "__amd_rocclr_copyBuffer", "FETCH_SIZE", 512), (2, "__amd_rocclr_copyBuffer",
+                                                                             "FETCH_SIZE", 512)])
+    cw = write("cw.csv", [(1, "__amd_rocclr_copyBuffer", "WRITE_SIZE", 1024)])
+    # k_gs: an early dispatch (dropped by last=2) and two timed ones, one split over two rows
+    kf = write("kf.csv", [(5, "void scx::k_gs<3, float, true>(scx::GSArgs<float>)", "FETCH_SIZE", 999),
+                          (7, "void scx::k_gs<3, float, true>(scx::GSArgs<float>)", "FETCH_SIZE", 100),
+                          (7, "void scx::k_gs<3, float, true>(scx::GSArgs<float>)", "FETCH_SIZE", 100),
+                          (8, "void scx::k_gs<3, double, true>(scx::GSArgs<double>)", "FETCH_SIZE", 400),
+                          (9, "void scx::k_colour(unsigned int)", "FETCH_SIZE", 7)])
+    kw = write("kw.csv", [(7, "void scx::k_gs<3, float, true>(scx::GSArgs<float>)", "WRITE_SIZE", 50),
+                          (8, "void scx::k_gs<3, double, true>(scx::GSArgs<double>)", "WRITE_SIZE", 150)])
+    t = rl.hbm_traffic(kf, kw, cf, cw, 1 << 20, "k_gs", last=2)
+    assert t["fetch_factor"] == 2.0 and t["write_factor"] == 1.0
+    assert t["dispatches"] == [2, 2]
+    assert t["fetch_bytes"] == 1024.0 * (200 + 400) / 2 * 2.0
+    assert t["write_bytes"] == 1024.0 * (50 + 150) / 2
+    assert t["bytes_per_launch"] == t["fetch_bytes"] + t["write_bytes"]
