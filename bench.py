@@ -7,8 +7,9 @@ tolerance 1e-6 (max over contacts of ||phi||^2; max ||phi|| is reported beside i
 FP32 sweeps + FP64 residual/polish, stepped with the learned warm start (a 16-128-128-3
 BF16 perceptron trained on the device from samples of the transported-cache solver).
 From the same settled state the plain coloured Gauss-Seidel (cold start, as the
-reference's 3D geometric map) and the transported impulse cache are measured as well; the
-sweep and body-step ratios go into config.warm_start_comparison.
+reference's 3D geometric map) and the transported impulse cache are measured as well
+(learned and transported step by step on the same checkpointed states of one trajectory);
+the sweep and body-step ratios go into config.warm_start_comparison.
 
 Other workloads (--config): cfg1, 1,048,576 RSA circles (2D, mu 0.3, dt 0.005, 100
 steps); cfg2, the pile with the plain coloured GS as the stepping mode; cfg4, an ensemble
@@ -109,33 +110,59 @@ def collect_and_train(ctx, s, prm, precision, args, log):
 
 
 def compare_warm_starts(ctx, s, prm, precision, args, log):
-    """The plain coloured Gauss-Seidel (cold start), the transported cache and the learned
-    warm start, each for a fixed number of steps from the same state (a transported-cache
-    step first primes the cache of every mode). The context ends in that state."""
+    """The transported cache and the learned warm start compared step by step on one
+    trajectory, and the plain coloured Gauss-Seidel (cold start) from its first state.
+
+    Two warm starts that both meet the tolerance leave slightly different states, and the
+    pile's trajectories diverge within a few steps, so means over two independent runs of
+    20 steps differ by about 10 % from that alone. Instead each of `compare_steps` states of
+    one reference trajectory (advanced by transported-cache steps) is restored from a
+    checkpoint (scx_save_checkpoint: bodies, impulse cache, model) and stepped once with
+    each warm start: paired sweep counts of the same step. The context ends in the state
+    it started from."""
     import scisim_amd as scx
-    tp = scx.step_params(precision=precision, **dict(prm, warm_start=scx.WARM_TRANSPORT))
-    q, v, w = ctx.get_state()
+
+    def params(mode):
+        return scx.step_params(precision=precision, **dict(prm, warm_start=mode))
+
+    img0 = ctx.save_checkpoint()
+    img = img0
     out = {}
-    plan = (("transport", scx.WARM_TRANSPORT, args.compare_steps), ("learned", scx.WARM_LEARNED, args.compare_steps),
-            ("cold", scx.WARM_NONE, args.cold_steps))
-    for name, mode, nsteps in plan:
-        if nsteps <= 0:
-            continue
-        ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
-        ctx.step(tp)
-        sw, ok = [], 0
+    paired = {"transport": [], "learned": []}
+    secs = {"transport": 0.0, "learned": 0.0}
+    ok = {"transport": 0, "learned": 0}
+    for k in range(args.compare_steps):
+        # learned first, then the transported-cache step that also advances the trajectory
+        for name, mode in (("learned", scx.WARM_LEARNED), ("transport", scx.WARM_TRANSPORT)):
+            ctx.load_checkpoint(img)
+            t0 = time.perf_counter()
+            st = ctx.step(params(mode))
+            secs[name] += time.perf_counter() - t0
+            paired[name].append(st["sweeps"])
+            ok[name] += int(st["succeeded"])
+        img = ctx.save_checkpoint()
+        log("compare step", k, "sweeps learned", paired["learned"][-1], "transport", paired["transport"][-1])
+    for name in ("transport", "learned"):
+        if paired[name]:
+            out[name] = {"steps": len(paired[name]), "mean_sweeps": float(np.mean(paired[name])),
+                         "steps_at_tolerance": ok[name], "body_steps_per_s": s["r"].size * len(paired[name]) / secs[name]}
+    if args.cold_steps > 0:
+        ctx.load_checkpoint(img0)
+        sw, okc = [], 0
         t0 = time.perf_counter()
-        for k in range(nsteps):
-            st = ctx.step(scx.step_params(precision=precision, **dict(prm, warm_start=mode)))
+        for k in range(args.cold_steps):
+            st = ctx.step(params(scx.WARM_NONE))
             sw.append(st["sweeps"])
-            ok += int(st["succeeded"])
-            log(name, "step", k, "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
-        dt = time.perf_counter() - t0
-        out[name] = {"steps": nsteps, "mean_sweeps": float(np.mean(sw)), "steps_at_tolerance": ok,
-                     "body_steps_per_s": s["r"].size * nsteps / dt}
-        log(name, out[name])
-    ctx.set_bodies(q, v, s["r"], s["m"], omega=w, inertia=s.get("inertia"))
-    ctx.step(tp)
+            okc += int(st["succeeded"])
+            log("cold step", k, "sweeps", st["sweeps"], "%.1fs" % (time.perf_counter() - t0))
+        out["cold"] = {"steps": args.cold_steps, "mean_sweeps": float(np.mean(sw)), "steps_at_tolerance": okc,
+                       "body_steps_per_s": s["r"].size * args.cold_steps / (time.perf_counter() - t0)}
+    ctx.load_checkpoint(img0)
+    if paired["learned"]:
+        r = np.asarray(paired["learned"], dtype=np.float64) / np.maximum(1.0, np.asarray(paired["transport"]))
+        se = float(r.std(ddof=1) / np.sqrt(r.size)) if r.size > 1 else float("nan")
+        out["paired_sweep_ratio_learned_over_transport"] = {"mean": float(r.mean()), "two_se": 2.0 * se,
+                                                            "steps": int(r.size)}
     if "cold" in out and "learned" in out:
         out["sweeps_cold_over_learned"] = out["cold"]["mean_sweeps"] / max(1e-9, out["learned"]["mean_sweeps"])
         out["body_steps_learned_over_cold"] = (out["learned"]["body_steps_per_s"] /
@@ -147,6 +174,9 @@ def compare_warm_starts(ctx, s, prm, precision, args, log):
                                                     out["transport"]["body_steps_per_s"])
     if "cold" in out and "transport" in out:
         out["sweeps_cold_over_transport"] = out["cold"]["mean_sweeps"] / max(1e-9, out["transport"]["mean_sweeps"])
+    for name in ("transport", "learned", "cold"):
+        if name in out:
+            log(name, out[name])
     return out
 
 
