@@ -247,6 +247,7 @@ void free_all(scx_ctx* c) {
   dfree(&d.scene_sweeps);
   dfree(&d.act_idx);
   dfree(&d.act_part);
+  dfree(&d.viol);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -460,6 +461,9 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     c->last_ncolours = d.ctr_host->ncolours;
+    if (c->violator_skip == 2 && p->precision == SCX_PRECISION_FP32_POLISH && c->nscenes == 0 &&
+        d.ctr_host->sweeps_fp32 >= 64 && d.ctr_host->evals_skipped == 0)
+      return fail(c, SCX_E_STATE, "no FP32 evaluation skipped in %u sweeps", d.ctr_host->sweeps_fp32);
     if (st) {
       st->nentries = nentries;
       st->npairs = nbb;
@@ -580,7 +584,12 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     // SCX_COHERENT_SWEEPS=0: the colour passes use ordinary body accesses and full grid
     // barriers (the same iterates bit for bit; for A/B measurements and tests)
     if (const char* env = getenv("SCX_COHERENT_SWEEPS")) c->coherent_sweeps = atoi(env) != 0;
+    // SCX_VIOLATOR_SKIP=0: every FP32 sweep gets a full residual evaluation (the same
+    // iterates and sweep counts; for A/B measurements and tests); 2: a step whose FP32 phase
+    // ran 64 sweeps or more without skipping one fails (tests)
+    if (const char* env = getenv("SCX_VIOLATOR_SKIP")) c->violator_skip = atoi(env);
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
+        (e = dalloc(&d.viol, (size_t)3 + kViolCap)) != hipSuccess ||
         (e = dalloc(&d.act_part, (size_t)kMaxColours * std::max(c->gs_blocks, 1))) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
       break;

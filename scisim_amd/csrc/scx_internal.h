@@ -18,6 +18,9 @@ constexpr int kScanItems = 8;         // items per thread in scans
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr uint32_t kNoBody = 0xFFFFFFFFu;
 constexpr int kMaxColours = 64;
+// Contacts listed per full residual evaluation of an FP32 phase (one per thread of the
+// sweep kernel's workgroup 0).
+constexpr uint32_t kViolCap = 512;
 constexpr uint32_t kMaxScenePlanes = 64;        // static planes of one scene (64-bit plane masks)
 constexpr uint32_t kMaxPlanesTotal = 1u << 24;  // static planes of a context (ensembles)
 
@@ -57,7 +60,7 @@ struct StepCounters {
   uint32_t sweeps;               // total sweeps
   uint32_t sweeps_fp32;
   uint32_t ncells;               // cells of this step's grid
-  uint32_t pad;
+  uint32_t evals_skipped;        // FP32 residual evaluations replaced by the violator check
   double error;                  // best residual of the last phase
   double error_fp32;
 };
@@ -187,6 +190,7 @@ struct DeviceBuffers {
   ContactSoA<float> c32{};
   double* rbest = nullptr;     // 3 per contact
   double* partial = nullptr;   // per-block residual partials
+  uint32_t* viol = nullptr;    // FP32 phase: 2 counts, 512 listed contacts, a flag (k_gs)
   // warm-start cache (previous step's active set, sorted keys + impulses)
   // keys ascend in active-set order: (i << 32 | j) for body pairs, then
   // (1 << 63 | plane << 32 | i) for plane contacts
@@ -253,6 +257,7 @@ struct scx_ctx {
   int force_blocks = 0;  // > 0: persistent kernels use this many workgroups (capped), for tests
   bool ens_compact = true;  // ensembles: sweep compacted lists of the unfrozen scenes' contacts
   bool coherent_sweeps = true;  // colour passes with coherent body accesses and light barriers
+  int violator_skip = 1;        // FP32 phase: skip full evaluations while a listed violator remains (2: required)
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
   // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
   // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need

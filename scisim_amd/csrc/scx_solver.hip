@@ -279,7 +279,14 @@ struct GSArgs {
   // per-workgroup counts for the compaction (kMaxColours x gridDim); nullptr: no compaction
   uint32_t* act_idx;
   uint32_t* act_part;
+  // FP32 phase of one scene: contacts whose residual at the last full evaluation was at
+  // least the capture threshold (up to kViolCap, in any order; counts of the two last
+  // evaluations, double-buffered), and the word through which workgroup 0 tells the others
+  // that one of them is still above the stopping value (nullptr: every sweep is evaluated)
+  uint32_t* viol;  // 2 counts, then kViolCap contacts
+  uint32_t* vflag;
 };
+
 
 // True if contact k belongs to a scene whose last evaluated residual is below stop.
 template <typename Real>
@@ -580,7 +587,7 @@ template <int D, typename Real, bool Coh = false>
 __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t gsize, uint32_t ev = 0,
                                 double stop = 0.0, uint32_t base = 0, uint32_t it = 0,
                                 const uint32_t* lst = nullptr, uint32_t nlst = 0,
-                                const BodyPort<Real>& P = BodyPort<Real>{}) {
+                                const BodyPort<Real>& P = BodyPort<Real>{}, double capture = -1.0) {
   // ensembles after a compaction: only the contacts of scenes not frozen are evaluated (a
   // frozen scene met the tolerance and is no longer updated; its buffer entry stays 0)
   const uint32_t nk = lst ? nlst : A.C;
@@ -622,9 +629,24 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
       }
     }
   } else if (A.phase == 1) {  // FP32 phase: an FP32 estimate (the FP64 phase decides on the tolerance)
-    // read-only loop: unrolled so that the loads of several contacts are in flight together
+    if (capture >= 0.0) {
+      // also list the contacts at or above `capture` (the violators that the sweeps after
+      // this evaluation check first); the other buffer's count starts the next list
+      if (gtid == 0) __hip_atomic_store(A.viol + ((ev + 1) & 1), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll 4
-    for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float, Coh>(A, k, P));
+      for (uint32_t k = gtid; k < A.C; k += gsize) {
+        const double e = (double)contact_err<D, Real, float, Coh>(A, k, P);
+        m = fmax(m, e);
+        if (e >= capture) {
+          const uint32_t i = atomicAdd(A.viol + (ev & 1), 1u);
+          if (i < kViolCap) __hip_atomic_store(A.viol + 2 + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    } else {
+      // read-only loop: unrolled so that the loads of several contacts are in flight together
+#pragma unroll 4
+      for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float, Coh>(A, k, P));
+    }
   } else {
 #pragma unroll 4
     for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real, double, Coh>(A, k, P));
@@ -728,7 +750,16 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   }
   uint32_t ev = 0;  // evaluations so far; the last one's scene residuals are in buffer (ev - 1) & 1
   const BodyPort<Real> P = body_port<D, Real>(A.body, A.nbodies);
-  double err = eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, 0, nullptr, 0, P);
+  // FP32 phase of one scene with a one-workgroup tail: after a full evaluation that listed
+  // the contacts near its maximum, a sweep's full evaluation is skipped while workgroup 0
+  // finds one of them still at or above `stop` (then the maximum is too, and the solve goes
+  // on exactly as the full evaluation would decide); a full evaluation at least every
+  // kFullEvery sweeps keeps the best residual for the stagnation test current
+  constexpr uint32_t kFullEvery = 256;
+  const bool subset = A.viol && A.phase == 1 && !A.cscene && min(A.tail_col, ncol) < ncol;
+  double err = eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, 0, nullptr, 0, P,
+                                           subset ? stop : -1.0);
+  uint32_t last_full = 0;
   const bool masked = A.cscene && A.nscenes <= kSceneMaskBits;
   if (masked) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
   // Ensembles: once enough scenes have frozen, the sweeps and evaluations walk a compacted
@@ -812,13 +843,42 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
             wait_stores();
             __syncthreads();
             if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+          }
+          if (subset && eval_now) {
+            // the listed contacts of the last full evaluation, one per thread: is one of
+            // them still at or above `stop`?
+            const uint32_t cnt = min(__hip_atomic_load(A.viol + ((ev - 1) & 1), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT), kViolCap);
+            double e = 0.0;
+            if (threadIdx.x < cnt)
+              e = (double)contact_err<D, Real, float, Coh>(
+                  A, __hip_atomic_load(A.viol + 2 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P);
+            e = wave_reduce_max_f64(e);
+            if ((threadIdx.x & 63) == 0) s_fm[threadIdx.x >> 6] = e;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+              double mx = s_fm[0];
+              for (uint32_t w = 1; w < kGSBlock / 64; ++w) mx = fmax(mx, s_fm[w]);
+              __hip_atomic_store(A.vflag, (cnt > 0 && mx >= stop) ? 1u : 0u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+            }
           }
         }
         if (Coh) grid_sync_coherent(A.bar);
         else grid_barrier(A.bar);
       }
+      if (eval_now && subset && it - last_full < kFullEvery &&
+          __hip_atomic_load(A.vflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u)
+      {
+        if (gtid == 0) ++A.ctr->evals_skipped;
+        continue;  // a listed contact is still at or above stop: not converged, no full evaluation
+      }
       if (eval_now) {
-        err = fmax(eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, it, lst, nlst, P), frozen_max);
+        last_full = it;
+        err = fmax(eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, it, lst, nlst, P,
+                                               subset ? fmax(stop, 0.1 * err) : -1.0),
+                   frozen_max);
         if (masked) build_frozen_mask(A, (ev - 1) & 1, stop, s_frozen);
         if (err < best) {
           best = err;
@@ -1415,6 +1475,13 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
                     reinterpret_cast<float*>(c->d.rbest), c->d.partial, c->d.ctr, c->d.barrier, p->max_iters,
                     p->eval_every, p->tol, 1, warm ? 1 : 0};
     A.lmap = c->d.lmap;
+    if (!ens && c->d.viol && c->violator_skip) {
+      // the violator list of the FP32 phase starts empty
+      e = hipMemsetAsync(c->d.viol, 0, sizeof(uint32_t) * (kViolCap + 3), s);
+      if (e != hipSuccess) return e;
+      A.viol = c->d.viol;
+      A.vflag = c->d.viol + 2 + kViolCap;
+    }
     if (ens) {
       A.cscene = c->d.cscene;
       A.scene_acc = c->d.scene_acc;

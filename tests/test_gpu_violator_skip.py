@@ -1,0 +1,82 @@
+"""The FP32 phase's shortcut past full residual evaluations.
+
+A full evaluation of the FP32 sweeps also lists the contacts whose residual is within a
+factor 10 of its maximum (at least the stopping value). While workgroup 0 finds one of
+them still at or above the stopping value after a sweep, the maximum over all contacts is
+too, so that sweep's full evaluation can only say "go on" and is skipped; once none is,
+the full evaluation runs and decides as before. The iterates, the sweep counts and the
+residuals must therefore be those of a full evaluation after every sweep
+(SCX_VIOLATOR_SKIP=0) bit for bit, and on these scenes some evaluations must actually be
+skipped (SCX_VIOLATOR_SKIP=2 fails a step whose FP32 phase ran 64 sweeps or more without
+skipping one).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import scisim_amd as scx
+from scisim_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def circles():
+    s = scenes.ball2d_rsa(n=20000, seed=7)
+    s["v"] = np.random.default_rng(7).normal(0.0, 2.0, s["v"].size)
+    return s
+
+
+@pytest.fixture(scope="module")
+def pile():
+    return scenes.spheres_pile(n=8000, seed=9)
+
+
+def _run(monkeypatch, s, skip, blocks, steps=5):
+    monkeypatch.setenv("SCX_VIOLATOR_SKIP", "2" if skip else "0")
+    if blocks:
+        monkeypatch.setenv("SCX_FORCE_PERSISTENT_BLOCKS", str(blocks))
+    else:
+        monkeypatch.delenv("SCX_FORCE_PERSISTENT_BLOCKS", raising=False)
+    D = s["dim"]
+    ctx = scx.Context(D, s["r"].size, max_planes=8)
+    try:
+        ctx.set_autotune(False)
+        ctx.set_planes(s["px"], s["pn"])
+        ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s.get("inertia"))
+        g = [0.0, -9.81] if D == 2 else [0.0, -9.81, 0.0]
+        prm = dict(dt=0.005, g=g, mu=0.4, CoR=0.0, tol=1e-6, max_iters=20000, eval_every=1,
+                   integrator=0 if D == 2 else 1, warm_start=scx.WARM_TRANSPORT, precision=scx.FP32_POLISH)
+        out = []
+        for k in range(steps):
+            st = ctx.step(scx.step_params(iteration=k + 1, **prm))
+            q, v, w = ctx.get_state()
+            con = ctx.contacts()
+            out.append((st, q, v, w, con["alpha"], con["beta"]))
+        return out
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("blocks", [16, 0])
+@pytest.mark.parametrize("kind", ["circles", "pile"])
+def test_skipped_evaluations_change_nothing(monkeypatch, request, kind, blocks):
+    s = request.getfixturevalue(kind)
+    _say("case", kind, "blocks", blocks)
+    ref = _run(monkeypatch, s, False, blocks)
+    got = _run(monkeypatch, s, True, blocks)
+    for k, (a, b) in enumerate(zip(ref, got)):
+        _say("step", k, "contacts", a[0]["ncontacts"], "colours", a[0]["ncolours"], "sweeps", a[0]["sweeps"],
+             b[0]["sweeps"], "fp32", a[0]["sweeps_fp32"], b[0]["sweeps_fp32"], "error", a[0]["error"], b[0]["error"])
+        for key in ("ncontacts", "ncolours", "sweeps", "sweeps_fp32", "error", "succeeded"):
+            assert a[0][key] == b[0][key], key
+        for x, y in zip(a[1:], b[1:]):
+            if x is not None:
+                assert np.array_equal(x, y)
