@@ -753,9 +753,11 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   // FP32 phase of one scene with a one-workgroup tail: after a full evaluation that listed
   // the contacts near its maximum, a sweep's full evaluation is skipped while workgroup 0
   // finds one of them still at or above `stop` (then the maximum is too, and the solve goes
-  // on exactly as the full evaluation would decide); a full evaluation at least every
-  // kFullEvery sweeps keeps the best residual for the stagnation test current
-  constexpr uint32_t kFullEvery = 256;
+  // on exactly as the full evaluation would decide). Only in the first kSkipUntil sweeps
+  // and with a full evaluation at least every kFullEvery: the stagnation test from sweep
+  // 1024 on compares best residuals, which every sweep from kSkipUntil on contributes to as
+  // before (a warm-started step converges well within kSkipUntil sweeps).
+  constexpr uint32_t kFullEvery = 256, kSkipUntil = 1024 - kFullEvery;
   const bool subset = A.viol && A.phase == 1 && !A.cscene && min(A.tail_col, ncol) < ncol;
   double err = eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, 0, nullptr, 0, P,
                                            subset ? stop : -1.0);
@@ -845,7 +847,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
             if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
           }
-          if (subset && eval_now) {
+          if (subset && eval_now && it < kSkipUntil) {
             // the listed contacts of the last full evaluation, one per thread: is one of
             // them still at or above `stop`?
             const uint32_t cnt = min(__hip_atomic_load(A.viol + ((ev - 1) & 1), __ATOMIC_RELAXED,
@@ -868,7 +870,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         if (Coh) grid_sync_coherent(A.bar);
         else grid_barrier(A.bar);
       }
-      if (eval_now && subset && it - last_full < kFullEvery &&
+      if (eval_now && subset && it < kSkipUntil && it - last_full < kFullEvery &&
           __hip_atomic_load(A.vflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u)
       {
         if (gtid == 0) ++A.ctr->evals_skipped;

@@ -8,7 +8,9 @@ the full evaluation runs and decides as before. The iterates, the sweep counts a
 residuals must therefore be those of a full evaluation after every sweep
 (SCX_VIOLATOR_SKIP=0) bit for bit, and on these scenes some evaluations must actually be
 skipped (SCX_VIOLATOR_SKIP=2 fails a step whose FP32 phase ran 64 sweeps or more without
-skipping one).
+skipping one). Skipping stops at sweep 768 of a launch, so that the FP32 phase's
+stagnation test (from sweep 1024 on) sees the same best residuals: the cold steps, whose
+FP32 phases run into it, must match as well.
 """
 
 import sys
@@ -39,7 +41,7 @@ def pile():
     return scenes.spheres_pile(n=8000, seed=9)
 
 
-def _run(monkeypatch, s, skip, blocks, steps=5):
+def _run(monkeypatch, s, skip, blocks, warm, steps=5):
     monkeypatch.setenv("SCX_VIOLATOR_SKIP", "2" if skip else "0")
     if blocks:
         monkeypatch.setenv("SCX_FORCE_PERSISTENT_BLOCKS", str(blocks))
@@ -52,8 +54,11 @@ def _run(monkeypatch, s, skip, blocks, steps=5):
         ctx.set_planes(s["px"], s["pn"])
         ctx.set_bodies(s["q"], s["v"], s["r"], s["m"], inertia=s.get("inertia"))
         g = [0.0, -9.81] if D == 2 else [0.0, -9.81, 0.0]
-        prm = dict(dt=0.005, g=g, mu=0.4, CoR=0.0, tol=1e-6, max_iters=20000, eval_every=1,
-                   integrator=0 if D == 2 else 1, warm_start=scx.WARM_TRANSPORT, precision=scx.FP32_POLISH)
+        # cold steps at a tight tolerance: long FP32 phases that end at their rounding floor
+        # through the stagnation test, then the FP64 polish
+        tol = 1e-6 if warm != scx.WARM_NONE else 1e-10
+        prm = dict(dt=0.005, g=g, mu=0.4, CoR=0.0, tol=tol, max_iters=20000, eval_every=1,
+                   integrator=0 if D == 2 else 1, warm_start=warm, precision=scx.FP32_POLISH)
         out = []
         for k in range(steps):
             st = ctx.step(scx.step_params(iteration=k + 1, **prm))
@@ -65,13 +70,16 @@ def _run(monkeypatch, s, skip, blocks, steps=5):
         ctx.close()
 
 
+@pytest.mark.parametrize("warm", [scx.WARM_TRANSPORT, scx.WARM_NONE], ids=["transport", "cold"])
 @pytest.mark.parametrize("blocks", [16, 0])
 @pytest.mark.parametrize("kind", ["circles", "pile"])
-def test_skipped_evaluations_change_nothing(monkeypatch, request, kind, blocks):
+def test_skipped_evaluations_change_nothing(monkeypatch, request, kind, blocks, warm):
     s = request.getfixturevalue(kind)
-    _say("case", kind, "blocks", blocks)
-    ref = _run(monkeypatch, s, False, blocks)
-    got = _run(monkeypatch, s, True, blocks)
+    _say("case", kind, "blocks", blocks, "warm", warm)
+    steps = 5 if warm != scx.WARM_NONE else 3
+    ref = _run(monkeypatch, s, False, blocks, warm, steps)
+    got = _run(monkeypatch, s, True, blocks, warm, steps)
+    long_fp32 = False
     for k, (a, b) in enumerate(zip(ref, got)):
         _say("step", k, "contacts", a[0]["ncontacts"], "colours", a[0]["ncolours"], "sweeps", a[0]["sweeps"],
              b[0]["sweeps"], "fp32", a[0]["sweeps_fp32"], b[0]["sweeps_fp32"], "error", a[0]["error"], b[0]["error"])
@@ -80,3 +88,6 @@ def test_skipped_evaluations_change_nothing(monkeypatch, request, kind, blocks):
         for x, y in zip(a[1:], b[1:]):
             if x is not None:
                 assert np.array_equal(x, y)
+        long_fp32 = long_fp32 or a[0]["sweeps_fp32"] > 1024
+    if warm == scx.WARM_NONE:
+        assert long_fp32  # a stagnation test ran
