@@ -123,6 +123,7 @@ int alloc_bodies(scx_ctx* c, uint32_t N) {
   HIPC(dalloc(&d.slot_of, N));
   HIPC(dalloc(&d.lmap, N));
   HIPC(hipMemset(d.lmap, 0xFF, sizeof(uint32_t) * N));
+  HIPC(dalloc(&d.comp_parent, N));
   d.sid = nullptr;
   d.slot_map = nullptr;
   HIPC(dalloc(&d.aabb_min, (size_t)D * N));
@@ -248,6 +249,9 @@ void free_all(scx_ctx* c) {
   dfree(&d.act_idx);
   dfree(&d.act_part);
   dfree(&d.viol);
+  dfree(&d.comp_parent);
+  dfree(&d.comp_cnt);
+  dfree(&d.comp_list);
   uint32_t* u32[] = {d.cells, d.ent, d.pcount, d.scan_tmp, d.pa, d.pb, d.ca, d.cb, d.mbcount, d.deg, d.adj,
                      d.perm, d.coloff};
   for (uint32_t* p : u32)
@@ -464,6 +468,8 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     if (c->violator_skip == 2 && p->precision == SCX_PRECISION_FP32_POLISH && c->nscenes == 0 &&
         d.ctr_host->sweeps_fp32 >= 64 && d.ctr_host->evals_skipped == 0)
       return fail(c, SCX_E_STATE, "no FP32 evaluation skipped in %u sweeps", d.ctr_host->sweeps_fp32);
+    if (c->comp_sweeps == 2 && c->nscenes == 0 && d.ctr_host->sweeps > 0 && d.ctr_host->comp_launches == 0)
+      return fail(c, SCX_E_STATE, "no sweep launch ran the component phase (%u colours)", d.ctr_host->ncolours);
     if (st) {
       st->nentries = nentries;
       st->npairs = nbb;
@@ -588,8 +594,15 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     // iterates and sweep counts; for A/B measurements and tests); 2: a step whose FP32 phase
     // ran 64 sweeps or more without skipping one fails (tests)
     if (const char* env = getenv("SCX_VIOLATOR_SKIP")) c->violator_skip = atoi(env);
+    // SCX_COMPONENT_SWEEPS=0: the late colours of one scene are swept in grid-wide passes and
+    // a one-workgroup tail instead of as components (the same iterates bit for bit; for A/B
+    // measurements and tests; 2: a step of one scene whose sweeps ran no component phase
+    // fails, for tests)
+    if (const char* env = getenv("SCX_COMPONENT_SWEEPS")) c->comp_sweeps = atoi(env);
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
         (e = dalloc(&d.viol, (size_t)3 + kViolCap)) != hipSuccess ||
+        (e = dalloc(&d.comp_cnt, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
+        (e = dalloc(&d.comp_list, (size_t)kCompCap * std::max(c->gs_blocks, 1))) != hipSuccess ||
         (e = dalloc(&d.act_part, (size_t)kMaxColours * std::max(c->gs_blocks, 1))) != hipSuccess) {
       rc = fail(c, SCX_E_HIP, "hipMalloc failed: %s", hipGetErrorString(e));
       break;

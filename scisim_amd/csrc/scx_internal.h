@@ -21,6 +21,8 @@ constexpr int kMaxColours = 64;
 // Contacts listed per full residual evaluation of an FP32 phase (one per thread of the
 // sweep kernel's workgroup 0).
 constexpr uint32_t kViolCap = 512;
+// Contacts of the component phase one workgroup of the persistent sweep kernel holds in LDS.
+constexpr uint32_t kCompCap = 2048;
 constexpr uint32_t kMaxScenePlanes = 64;        // static planes of one scene (64-bit plane masks)
 constexpr uint32_t kMaxPlanesTotal = 1u << 24;  // static planes of a context (ensembles)
 
@@ -61,6 +63,8 @@ struct StepCounters {
   uint32_t sweeps_fp32;
   uint32_t ncells;               // cells of this step's grid
   uint32_t evals_skipped;        // FP32 residual evaluations replaced by the violator check
+  uint32_t comp_launches;        // sweep launches that ran the component phase
+  uint32_t pad;
   double error;                  // best residual of the last phase
   double error_fp32;
 };
@@ -191,6 +195,9 @@ struct DeviceBuffers {
   double* rbest = nullptr;     // 3 per contact
   double* partial = nullptr;   // per-block residual partials
   uint32_t* viol = nullptr;    // FP32 phase: 2 counts, 512 listed contacts, a flag (k_gs)
+  uint32_t* comp_parent = nullptr;  // N: union-find parents of the component phase (k_gs)
+  uint32_t* comp_cnt = nullptr;     // 1 + gs_blocks: overflow flag, per-workgroup counts
+  uint32_t* comp_list = nullptr;    // gs_blocks x kCompCap: per-workgroup contacts
   // warm-start cache (previous step's active set, sorted keys + impulses)
   // keys ascend in active-set order: (i << 32 | j) for body pairs, then
   // (1 << 63 | plane << 32 | i) for plane contacts
@@ -258,6 +265,7 @@ struct scx_ctx {
   bool ens_compact = true;  // ensembles: sweep compacted lists of the unfrozen scenes' contacts
   bool coherent_sweeps = true;  // colour passes with coherent body accesses and light barriers
   int violator_skip = 1;        // FP32 phase: skip full evaluations while a listed violator remains (2: required)
+  int comp_sweeps = 1;          // one scene: sweep the late colours as components, one workgroup each (2: required)
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
   // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
   // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need

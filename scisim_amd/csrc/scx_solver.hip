@@ -285,6 +285,16 @@ struct GSArgs {
   // that one of them is still above the stopping value (nullptr: every sweep is evaluated)
   uint32_t* viol;  // 2 counts, then kViolCap contacts
   uint32_t* vflag;
+  // One scene with coherent passes on several workgroups: colours [comp_col, ncolours) are
+  // swept as connected components (contacts linked through shared bodies), every component
+  // by one workgroup through its L2. comp_parent: per body slot, its parent in a union-find
+  // forest over the bodies of those contacts; comp_cnt: an overflow flag, then per
+  // workgroup the number of contacts it was given; comp_list: per workgroup kCompCap
+  // contacts. comp_parent == nullptr: no component phase.
+  uint32_t* comp_parent;
+  uint32_t* comp_cnt;
+  uint32_t* comp_list;
+  uint32_t comp_col;
 };
 
 
@@ -715,6 +725,134 @@ __device__ void apply_impulses(const GSArgs<Real>& A, const uint32_t* off, uint3
   }
 }
 
+// Union-find over body slots for the component phase. Every access to the parent array is
+// an agent-scope atomic (the workgroups of all XCDs share it). A parent is always a smaller
+// slot: a root is linked only under a smaller root (compare-and-swap, which fails when
+// another thread linked it first), and path halving replaces a parent by the parent's parent,
+// another ancestor; so the forest stays acyclic under any interleaving, and once every link
+// is made (after a grid barrier) a find returns the component's smallest slot.
+__device__ __forceinline__ uint32_t uf_ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
+  for (;;) {
+    const uint32_t p = uf_ld(parent + x);
+    if (p == x) return x;
+    const uint32_t gp = uf_ld(parent + p);
+    if (gp == p) return p;
+    __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = gp;
+  }
+}
+__device__ __forceinline__ void uf_union(uint32_t* parent, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    if (a > b) {
+      const uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t expect = b;
+    if (__hip_atomic_compare_exchange_strong(parent + b, &expect, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+// Component phase set-up, once per launch (one scene, coherent passes, several workgroups):
+// the contacts of colours [first, ncol) are grouped into connected components through their
+// bodies, every component is given to the workgroup hash(root) % gridDim.x, and each
+// workgroup lists its contacts in LDS sorted by colour (s_list, colour offsets s_coff).
+// The order inside a colour is free: contacts of one colour share no body. Returns `first`,
+// or ncol (no component phase) when some workgroup was given more than kCompCap contacts.
+template <int D, typename Real>
+__device__ uint32_t component_setup(const GSArgs<Real>& A, const uint32_t* off, uint32_t ncol, uint32_t first,
+                                    uint32_t gtid, uint32_t gsize, uint32_t* s_list, uint32_t* s_coff) {
+  uint32_t* par = A.comp_parent;
+  const uint32_t k0 = off[first], k1 = off[ncol];
+  for (uint32_t k = k0 + gtid; k < k1; k += gsize) {
+    const uint32_t a = A.c.a[k], b = A.c.b[k];
+    __hip_atomic_store(par + a, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (b != kNoBody) __hip_atomic_store(par + b, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (uint32_t w = gtid; w <= gridDim.x; w += gsize)
+    __hip_atomic_store(A.comp_cnt + w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  grid_sync_coherent(A.bar);
+  for (uint32_t k = k0 + gtid; k < k1; k += gsize)
+    if (A.c.b[k] != kNoBody) uf_union(par, A.c.a[k], A.c.b[k]);
+  grid_sync_coherent(A.bar);
+  for (uint32_t k = k0 + gtid; k < k1; k += gsize) {
+    const uint32_t w = fmix32(uf_find(par, A.c.a[k])) % gridDim.x;
+    const uint32_t pos = __hip_atomic_fetch_add(A.comp_cnt + 1 + w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (pos < kCompCap)
+      __hip_atomic_store(A.comp_list + (size_t)w * kCompCap + pos, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      __hip_atomic_store(A.comp_cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  grid_sync_coherent(A.bar);
+  if (__hip_atomic_load(A.comp_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return ncol;
+  // counting sort of this workgroup's contacts by colour
+  const uint32_t n = __hip_atomic_load(A.comp_cnt + 1 + blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t* src = A.comp_list + (size_t)blockIdx.x * kCompCap;
+  __shared__ uint32_t s_pos[kMaxColours + 1];
+  for (uint32_t c = threadIdx.x; c <= kMaxColours; c += kGSBlock) s_pos[c] = 0u;
+  __syncthreads();
+  // colour of contact k: the last c in [first, ncol) with off[c] <= k
+  auto colour_of = [&](uint32_t k) {
+    uint32_t lo = first, hi = ncol - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  };
+  for (uint32_t j = threadIdx.x; j < n; j += kGSBlock)
+    atomicAdd(s_pos + colour_of(__hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)), 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (uint32_t c = first; c < ncol; ++c) {
+      const uint32_t x = s_pos[c];
+      s_coff[c] = s_pos[c] = sum;
+      sum += x;
+    }
+    s_coff[ncol] = sum;
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < n; j += kGSBlock) {
+    const uint32_t k = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_list[atomicAdd(s_pos + colour_of(k), 1u)] = k;
+  }
+  __syncthreads();
+  return first;
+}
+
+// FP32 phase, workgroup 0: is one of the contacts listed by the last full evaluation (one
+// per thread) still at or above `stop`? The answer goes to A.vflag for every workgroup
+// after the next grid barrier.
+template <int D, typename Real, bool Coh>
+__device__ void violator_check(const GSArgs<Real>& A, uint32_t ev, double stop, const BodyPort<Real>& P,
+                               double* s_fm) {
+  const uint32_t cnt =
+      min(__hip_atomic_load(A.viol + ((ev - 1) & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), kViolCap);
+  double e = 0.0;
+  if (threadIdx.x < cnt)
+    e = (double)contact_err<D, Real, float, Coh>(
+        A, __hip_atomic_load(A.viol + 2 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P);
+  e = wave_reduce_max_f64(e);
+  if ((threadIdx.x & 63) == 0) s_fm[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = s_fm[0];
+    for (uint32_t w = 1; w < kGSBlock / 64; ++w) mx = fmax(mx, s_fm[w]);
+    __hip_atomic_store(A.vflag, (cnt > 0 && mx >= stop) ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
 // Coh: the colour passes and the residual evaluations access the body records and the
 // impulses coherently (BodyPort, ld_imp/st_imp) and are separated by grid_sync_coherent;
 // the small colours that workgroup 0 sweeps alone go through its L2 between an agent-scope
@@ -750,6 +888,17 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   }
   uint32_t ev = 0;  // evaluations so far; the last one's scene residuals are in buffer (ev - 1) & 1
   const BodyPort<Real> P = body_port<D, Real>(A.body, A.nbodies);
+  // Component phase: after the wide colours [0, ccol), each workgroup sweeps the components
+  // it was given colour by colour, through its L2 between an agent-scope acquire and
+  // release, and one grid barrier closes the sweep (instead of one per colour and a
+  // one-workgroup tail). Components share no body, so this is the sweep order of the
+  // colour passes for every contact.
+  __shared__ uint32_t s_clist[kCompCap];
+  __shared__ uint32_t s_coff[kMaxColours + 1];
+  uint32_t ccol = ncol;
+  if (Coh && A.comp_parent && !A.cscene && gridDim.x > 1 && A.comp_col < ncol)
+    ccol = component_setup<D, Real>(A, off, ncol, A.comp_col, gtid, gsize, s_clist, s_coff);
+  if (ccol < ncol && gtid == 0) ++A.ctr->comp_launches;
   // FP32 phase of one scene with a one-workgroup tail: after a full evaluation that listed
   // the contacts near its maximum, a sweep's full evaluation is skipped while workgroup 0
   // finds one of them still at or above `stop` (then the maximum is too, and the solve goes
@@ -758,7 +907,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   // 1024 on compares best residuals, which every sweep from kSkipUntil on contributes to as
   // before (a warm-started step converges well within kSkipUntil sweeps).
   constexpr uint32_t kFullEvery = 256, kSkipUntil = 1024 - kFullEvery;
-  const bool subset = A.viol && A.phase == 1 && !A.cscene && min(A.tail_col, ncol) < ncol;
+  const bool subset = A.viol && A.phase == 1 && !A.cscene && (min(A.tail_col, ncol) < ncol || ccol < ncol);
   double err = eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, 0, nullptr, 0, P,
                                            subset ? stop : -1.0);
   uint32_t last_full = 0;
@@ -792,13 +941,15 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       copy_pending = false;
       const uint32_t eb = (ev - 1) & 1;
       const uint32_t tail = min(A.tail_col, ncol);
+      const uint32_t nwide = (ccol < ncol) ? ccol : tail;  // colours swept in grid-wide passes
       const bool eval_now = A.eval_every > 0 && it % A.eval_every == 0;
+      const bool may_skip = subset && eval_now && it < kSkipUntil && it - last_full < kFullEvery;
       // the first contact of each colour pass is loaded before the barrier that opens it
       // completes (between the arrival and the wait of a coherent barrier), so that only the
       // body gathers wait for the barrier
       Loaded<D, Real> next;
-      if (tail > 0 && o[0] + gtid < o[1]) load_contact<D, Real, Coh>(A, lst ? lst[o[0] + gtid] : o[0] + gtid, next);
-      for (uint32_t col = 0; col < tail; ++col) {
+      if (nwide > 0 && o[0] + gtid < o[1]) load_contact<D, Real, Coh>(A, lst ? lst[o[0] + gtid] : o[0] + gtid, next);
+      for (uint32_t col = 0; col < nwide; ++col) {
         uint32_t j = o[col] + gtid;
         if (j < o[col + 1]) {
           uint32_t k = lst ? lst[j] : j;
@@ -810,7 +961,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
             else process<D, Real, Coh>(A, k, false, copy, P);
           }
         }
-        const bool pre = col + 1 < tail && o[col + 1] + gtid < o[col + 2];
+        const bool pre = col + 1 < nwide && o[col + 1] + gtid < o[col + 2];
         if (Coh) {
           const GridTicket t = grid_arrive_coherent(A.bar);
           if (pre) load_contact<D, Real, Coh>(A, lst ? lst[o[col + 1] + gtid] : o[col + 1] + gtid, next);
@@ -820,7 +971,27 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
           grid_barrier(A.bar);
         }
       }
-      if (tail < ncol) {
+      if (ccol < ncol) {
+        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __syncthreads();
+        for (uint32_t col = ccol; col < ncol; ++col) {
+          const uint32_t lo = s_coff[col], hi = s_coff[col + 1];
+          if (lo == hi) continue;  // the same in every thread of the workgroup
+          for (uint32_t j = lo + threadIdx.x; j < hi; j += kGSBlock) process<D, Real>(A, s_clist[j], false, copy);
+          __syncthreads();
+        }
+        wait_stores();
+        __syncthreads();
+        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        grid_sync_coherent(A.bar);
+        if (may_skip) {
+          // the listed contacts may lie in any workgroup's components: checked after the
+          // barrier, and one more barrier publishes the answer
+          if (blockIdx.x == 0) violator_check<D, Real, Coh>(A, ev, stop, P, s_fm);
+          grid_sync_coherent(A.bar);
+        }
+      } else if (tail < ncol) {
         // the last, small colours (typically the contacts of a few bodies of high degree):
         // one workgroup sweeps them in order with workgroup barriers, then one grid barrier,
         // instead of a grid barrier per colour; the sweep order is unchanged
@@ -847,32 +1018,12 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
             if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
           }
-          if (subset && eval_now && it < kSkipUntil) {
-            // the listed contacts of the last full evaluation, one per thread: is one of
-            // them still at or above `stop`?
-            const uint32_t cnt = min(__hip_atomic_load(A.viol + ((ev - 1) & 1), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT), kViolCap);
-            double e = 0.0;
-            if (threadIdx.x < cnt)
-              e = (double)contact_err<D, Real, float, Coh>(
-                  A, __hip_atomic_load(A.viol + 2 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P);
-            e = wave_reduce_max_f64(e);
-            if ((threadIdx.x & 63) == 0) s_fm[threadIdx.x >> 6] = e;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-              double mx = s_fm[0];
-              for (uint32_t w = 1; w < kGSBlock / 64; ++w) mx = fmax(mx, s_fm[w]);
-              __hip_atomic_store(A.vflag, (cnt > 0 && mx >= stop) ? 1u : 0u, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-            }
-          }
+          if (may_skip) violator_check<D, Real, Coh>(A, ev, stop, P, s_fm);
         }
         if (Coh) grid_sync_coherent(A.bar);
         else grid_barrier(A.bar);
       }
-      if (eval_now && subset && it < kSkipUntil && it - last_full < kFullEvery &&
-          __hip_atomic_load(A.vflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u)
-      {
+      if (may_skip && __hip_atomic_load(A.vflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
         if (gtid == 0) ++A.ctr->evals_skipped;
         continue;  // a listed contact is still at or above stop: not converged, no full evaluation
       }
@@ -1382,17 +1533,31 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
   int tail = ncol;
   while (tail > 0 && col_off[tail] - col_off[tail - 1] <= (uint32_t)kGSBlock) --tail;
   if (ncol - tail < 2) tail = ncol;
-  // per sweep: one barrier interval per colour before the tail, one for the tail and one
-  // for the residual over all contacts
-  uint32_t widths[kMaxColours + 2];
-  int nw = 0;
-  for (int b = 0; b < tail; ++b)
-    if (col_off[b + 1] > col_off[b]) widths[nw++] = col_off[b + 1] - col_off[b];
-  if (tail < ncol) widths[nw++] = 1;
-  widths[nw++] = A.C;
   // coherent colour passes unless switched off or the body records exceed the 32-bit
   // byte offsets of the buffer accesses
   const bool coh = c->coherent_sweeps && (uint64_t)A.nbodies * kBodyVecs<D> * 4 * sizeof(Real) < (1ull << 31);
+  // Component phase (one scene, coherent passes): the colours after which at most 15% of
+  // the contacts remain. In the contact graphs of packings and piles (a body touches about
+  // six others) the contacts of those colours form many small components, of a few dozen
+  // contacts at most; the phase starts before the tail and spans two colours at least.
+  int comp = ncol;
+  uint32_t comp_contacts = 0;
+  if (coh && A.comp_parent && !A.cscene) {
+    int k = ncol;
+    while (k > 0 && (uint64_t)(col_off[ncol] - col_off[k - 1]) * 100 <= (uint64_t)A.C * 15) --k;
+    if (k < tail && ncol - k >= 2) {
+      comp = k;
+      comp_contacts = col_off[ncol] - col_off[k];
+    }
+  }
+  // per sweep: one barrier interval per colour before the tail (or the component phase),
+  // one for the tail (or the component phase) and one for the residual over all contacts
+  uint32_t widths[kMaxColours + 2];
+  int nw = 0;
+  for (int b = 0; b < std::min(tail, comp); ++b)
+    if (col_off[b + 1] > col_off[b]) widths[nw++] = col_off[b + 1] - col_off[b];
+  if (std::min(tail, comp) < ncol) widths[nw++] = 1;
+  widths[nw++] = A.C;
   const int model =
       c->force_blocks > 0 ? c->force_blocks : persistent_blocks(widths, nw, kGSBlock, max_blocks, coh);
   const size_t lds = LdsLayout<D, Real>::bytes(A.C);
@@ -1405,6 +1570,11 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
                         : model;
   const int blocks = c->coop_unavailable ? 1 : std::max(1, std::min(max_blocks, want));
   A.tail_col = (blocks > 1) ? (uint32_t)tail : (uint32_t)kMaxColours;
+  // the component phase needs several workgroups, and lists that hold the phase's contacts
+  // with room for an uneven split (else the tail as before; a launch whose split still
+  // overflows a list falls back to it by itself)
+  if (blocks > 1 && comp < ncol && comp_contacts <= (uint64_t)blocks * kCompCap / 2) A.comp_col = (uint32_t)comp;
+  else A.comp_parent = nullptr;
   if (lds_path) {
     // the whole solve in one workgroup's LDS
     hipError_t e = hipFuncSetAttribute((const void*)k_gs_lds<D, Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1484,6 +1654,11 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
       A.viol = c->d.viol;
       A.vflag = c->d.viol + 2 + kViolCap;
     }
+    if (!ens && c->comp_sweeps) {
+      A.comp_parent = c->d.comp_parent;
+      A.comp_cnt = c->d.comp_cnt;
+      A.comp_list = c->d.comp_list;
+    }
     if (ens) {
       A.cscene = c->d.cscene;
       A.scene_acc = c->d.scene_acc;
@@ -1507,6 +1682,11 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
   B.lmap = c->d.lmap;
+  if (!ens && c->comp_sweeps) {
+    B.comp_parent = c->d.comp_parent;
+    B.comp_cnt = c->d.comp_cnt;
+    B.comp_list = c->d.comp_list;
+  }
   if (ens) {
     B.cscene = c->d.cscene;
     B.scene_acc = c->d.scene_acc;
