@@ -368,6 +368,7 @@ def main():
     phase_ms = {}
     pairs_bytes = gs_bytes = bp_bytes = 0.0
     contacts, sweeps, errors = [], [], []
+    evals_full = evals_skipped = 0
     fp32 = precision == scx.FP32_POLISH
     step_ms = []
     converged = 0
@@ -407,9 +408,15 @@ def main():
             gs_bytes += float(np.sum(cnt_k * (sw64 * rl.gs_sweep_bytes(dim, False) +
                                               (sw64 + 1) * rl.gs_eval_bytes(dim, False))))
         else:
+            # evaluations as run: the FP32 phase replaces most of them by a check of the
+            # contacts the last full one listed (a few hundred contacts, not counted)
+            ev = ctx.sweep_counters()
+            evals_full += ev["full"]
+            evals_skipped += ev["skipped"]
+            e32 = ev["full_fp32"] if fp32 else 0
             if fp32:
-                gs_bytes += rl.gs_launch_bytes(dim, C, s32, s32 + 1, True)
-            gs_bytes += rl.gs_launch_bytes(dim, C, s64, s64 + 1, False)
+                gs_bytes += rl.gs_launch_bytes(dim, C, s32, e32, True)
+            gs_bytes += rl.gs_launch_bytes(dim, C, s64, ev["full"] - e32, False)
         contacts.append(C)
         sweeps.append(st["sweeps"])
         errors.append(st["error"])
@@ -462,6 +469,9 @@ def main():
                   max_iters=prm["max_iters"], steps_at_tolerance=converged,
                   residual_max_phi_squared=emax, residual_max_phi=float(np.sqrt(emax)),
                   phase_ms_per_step={k: v / args.steps for k, v in phase_ms.items()})
+    if "ensemble" not in s:
+        config["residual_evaluations_per_step"] = {"full": evals_full / args.steps,
+                                                   "skipped": evals_skipped / args.steps}
     if training:
         config["warm_start_training"] = training
     if comparison:

@@ -282,6 +282,15 @@ int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const d
 int scx_set_timing(scx_ctx* ctx, int32_t enable);
 int scx_get_timing(scx_ctx* ctx, uint32_t cap, const char** names, double* ms, uint32_t* count);
 
+/* Counters of the last step that returned stats: residual evaluations that were full passes
+ * over the contacts (all phases, and of those the FP32 phase's), FP32 evaluations replaced
+ * by the check of the contacts the last full one listed as near its maximum, and sweep
+ * launches that swept their late colours as components. No counterpart in the reference
+ * (which evaluates after every eval_every-th sweep); they give the sweep kernel's byte count
+ * and show which paths a step took. Any pointer may be NULL. */
+int scx_get_sweep_counters(const scx_ctx* ctx, uint32_t* full_evals, uint32_t* full_evals_fp32,
+                           uint32_t* skipped_evals, uint32_t* component_launches);
+
 /* Online autotuning of the persistent Gauss-Seidel launches (no counterpart in the
  * reference, whose So-bogus solve has no launch shape; this is for callers that step
  * one kind of scene many times). Enabled, each sweep launch runs on the cost model's

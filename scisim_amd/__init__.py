@@ -36,7 +36,7 @@ STAGGER_STABILIZED = 2
 # Every symbol include/scx.h declares.
 EXPORTS = ["scx_version", "scx_last_error", "scx_create", "scx_destroy", "scx_potential_overlaps", "scx_set_bodies",
            "scx_set_planes", "scx_get_state", "scx_step", "scx_run", "scx_get_contacts", "scx_get_v2",
-           "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing",
+           "scx_friction_solve", "scx_unconstrained_flow", "scx_set_timing", "scx_get_timing", "scx_get_sweep_counters",
            "scx_set_warm_model", "scx_get_warm_model", "scx_get_samples", "scx_warm_model_gradients",
            "scx_train_warm_model", "scx_set_scenes", "scx_get_scene_stats", "scx_set_autotune", "scx_get_autotune",
            "scx_save_checkpoint", "scx_load_checkpoint", "scx_checkpoint_info", "scx_get_contact_forces", "scx_potential_overlaps_noalloc"]
@@ -118,6 +118,7 @@ def lib():
         L.scx_unconstrained_flow.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint32, _pd, _pd, _d, _pd, _pd, _pd, _pd]
         L.scx_set_timing.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_timing.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p), _pd, _pu32]
+        L.scx_get_sweep_counters.argtypes = [_vp, _pu32, _pu32, _pu32, _pu32]
         L.scx_set_autotune.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_autotune.argtypes = [_vp, ctypes.c_uint32, _pu32, _pi32, _pi32, _pf, _pi32, _pu32]
         L.scx_set_warm_model.argtypes = [_vp, _pf, ctypes.c_uint32]
@@ -357,6 +358,14 @@ class Context:
         ms = np.zeros(32)
         self._check(lib().scx_get_timing(self._h, 32, names, _p(ms), ctypes.byref(cnt)), "scx_get_timing")
         return [(names[i].decode(), float(ms[i])) for i in range(min(cnt.value, 32))]
+
+    def sweep_counters(self):
+        """Counters of the last step: full residual evaluations (all phases and the FP32
+        phase's), FP32 evaluations replaced by the check of the listed contacts, and sweep
+        launches that ran the component phase."""
+        x = [ctypes.c_uint32(0) for _ in range(4)]
+        self._check(lib().scx_get_sweep_counters(self._h, *[ctypes.byref(v) for v in x]), "scx_get_sweep_counters")
+        return dict(zip(("full", "full_fp32", "skipped", "component_launches"), (v.value for v in x)))
 
     def set_autotune(self, on):
         """Online autotuning of the sweep launch width (results do not depend on it)."""

@@ -468,8 +468,6 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     if (c->violator_skip == 2 && p->precision == SCX_PRECISION_FP32_POLISH && c->nscenes == 0 &&
         d.ctr_host->sweeps_fp32 >= 64 && d.ctr_host->evals_skipped == 0)
       return fail(c, SCX_E_STATE, "no FP32 evaluation skipped in %u sweeps", d.ctr_host->sweeps_fp32);
-    if (c->comp_sweeps == 2 && c->nscenes == 0 && d.ctr_host->sweeps > 0 && d.ctr_host->comp_launches == 0)
-      return fail(c, SCX_E_STATE, "no sweep launch ran the component phase (%u colours)", d.ctr_host->ncolours);
     if (st) {
       st->nentries = nentries;
       st->npairs = nbb;
@@ -596,9 +594,11 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     if (const char* env = getenv("SCX_VIOLATOR_SKIP")) c->violator_skip = atoi(env);
     // SCX_COMPONENT_SWEEPS=0: the late colours of one scene are swept in grid-wide passes and
     // a one-workgroup tail instead of as components (the same iterates bit for bit; for A/B
-    // measurements and tests; 2: a step of one scene whose sweeps ran no component phase
-    // fails, for tests)
-    if (const char* env = getenv("SCX_COMPONENT_SWEEPS")) c->comp_sweeps = atoi(env);
+    // measurements and tests)
+    if (const char* env = getenv("SCX_COMPONENT_SWEEPS")) c->comp_sweeps = atoi(env) != 0;
+    // SCX_COMPONENT_PERCENT: the share of the contacts (in percent, 1 to 50) in the colours
+    // of the component phase at most (default 15)
+    if (const char* env = getenv("SCX_COMPONENT_PERCENT")) c->comp_percent = std::min(50, std::max(1, atoi(env)));
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
         (e = dalloc(&d.viol, (size_t)3 + kViolCap)) != hipSuccess ||
         (e = dalloc(&d.comp_cnt, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
@@ -1594,6 +1594,18 @@ int scx_get_timing(scx_ctx* c, uint32_t cap, const char** names, double* ms, uin
     if (names) names[i] = c->phase_names[i];
     if (ms) ms[i] = c->phase_ms[i];
   }
+  return SCX_OK;
+}
+
+int scx_get_sweep_counters(const scx_ctx* c, uint32_t* full_evals, uint32_t* full_evals_fp32,
+                           uint32_t* skipped_evals, uint32_t* component_launches) {
+  if (!c) return SCX_E_ARG;
+  // the step's counters, copied to the host at the end of a step that returned stats
+  const StepCounters* h = c->d.ctr_host;
+  if (full_evals) *full_evals = h ? h->evals : 0u;
+  if (full_evals_fp32) *full_evals_fp32 = h ? h->evals_fp32 : 0u;
+  if (skipped_evals) *skipped_evals = h ? h->evals_skipped : 0u;
+  if (component_launches) *component_launches = h ? h->comp_launches : 0u;
   return SCX_OK;
 }
 

@@ -64,6 +64,8 @@ struct StepCounters {
   uint32_t ncells;               // cells of this step's grid
   uint32_t evals_skipped;        // FP32 residual evaluations replaced by the violator check
   uint32_t comp_launches;        // sweep launches that ran the component phase
+  uint32_t evals;                // full residual evaluations (passes over the contacts), all phases
+  uint32_t evals_fp32;           // of which in the FP32 phase
   uint32_t pad;
   double error;                  // best residual of the last phase
   double error_fp32;
@@ -265,7 +267,8 @@ struct scx_ctx {
   bool ens_compact = true;  // ensembles: sweep compacted lists of the unfrozen scenes' contacts
   bool coherent_sweeps = true;  // colour passes with coherent body accesses and light barriers
   int violator_skip = 1;        // FP32 phase: skip full evaluations while a listed violator remains (2: required)
-  int comp_sweeps = 1;          // one scene: sweep the late colours as components, one workgroup each (2: required)
+  bool comp_sweeps = true;      // one scene: sweep the late colours as components, one workgroup each
+  int comp_percent = 15;        // the component phase starts where at most this share of the contacts remains
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
   // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
   // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need

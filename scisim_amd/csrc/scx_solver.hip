@@ -1091,9 +1091,11 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
     if (A.phase == 1) {
       A.ctr->sweeps_fp32 = it;
       A.ctr->error_fp32 = best;
+      A.ctr->evals_fp32 = ev;
     }
     A.ctr->sweeps += it;
     A.ctr->error = best;
+    A.ctr->evals += ev;
   }
 }
 
@@ -1320,6 +1322,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
   const bool track_best = A.phase != 1;
   if (A.apply_init) apply_all(false);
   double err = residual();
+  uint32_t nev = 1;  // full residual evaluations
   double best = err;
   if (track_best)
     for (uint32_t k = tid; k < C; k += kGSBlock)
@@ -1337,6 +1340,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
       }
       if (A.eval_every > 0 && it % A.eval_every == 0) {
         err = residual();
+        ++nev;
         if (err < best) {
           best = err;
           if (track_best)
@@ -1380,9 +1384,11 @@ __global__ void __launch_bounds__(kGSBlock) k_gs_lds(GSArgs<Real> A) {
     if (A.phase == 1) {
       A.ctr->sweeps_fp32 = it;
       A.ctr->error_fp32 = best;
+      A.ctr->evals_fp32 = nev;
     }
     A.ctr->sweeps += it;
     A.ctr->error = best;
+    A.ctr->evals += nev;
   }
 }
 
@@ -1537,14 +1543,14 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
   // byte offsets of the buffer accesses
   const bool coh = c->coherent_sweeps && (uint64_t)A.nbodies * kBodyVecs<D> * 4 * sizeof(Real) < (1ull << 31);
   // Component phase (one scene, coherent passes): the colours after which at most 15% of
-  // the contacts remain. In the contact graphs of packings and piles (a body touches about
+  // the contacts remain (scx_ctx::comp_percent). In the contact graphs of packings and piles (a body touches about
   // six others) the contacts of those colours form many small components, of a few dozen
   // contacts at most; the phase starts before the tail and spans two colours at least.
   int comp = ncol;
   uint32_t comp_contacts = 0;
   if (coh && A.comp_parent && !A.cscene) {
     int k = ncol;
-    while (k > 0 && (uint64_t)(col_off[ncol] - col_off[k - 1]) * 100 <= (uint64_t)A.C * 15) --k;
+    while (k > 0 && (uint64_t)(col_off[ncol] - col_off[k - 1]) * 100 <= (uint64_t)A.C * c->comp_percent) --k;
     if (k < tail && ncol - k >= 2) {
       comp = k;
       comp_contacts = col_off[ncol] - col_off[k];

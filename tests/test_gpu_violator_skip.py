@@ -10,7 +10,9 @@ residuals must therefore be those of a full evaluation after every sweep
 skipped (SCX_VIOLATOR_SKIP=2 fails a step whose FP32 phase ran 64 sweeps or more without
 skipping one). Skipping stops at sweep 768 of a launch, so that the FP32 phase's
 stagnation test (from sweep 1024 on) sees the same best residuals: the cold steps, whose
-FP32 phases run into it, must match as well.
+FP32 phases run into it, must match as well. The evaluation counters (scx_get_sweep_counters)
+must add up: one evaluation before the first sweep of each phase and one after every sweep
+(eval_every = 1), less the skipped ones, which the run without skipping performs in full.
 """
 
 import sys
@@ -64,7 +66,7 @@ def _run(monkeypatch, s, skip, blocks, warm, steps=5):
             st = ctx.step(scx.step_params(iteration=k + 1, **prm))
             q, v, w = ctx.get_state()
             con = ctx.contacts()
-            out.append((st, q, v, w, con["alpha"], con["beta"]))
+            out.append((st, ctx.sweep_counters(), q, v, w, con["alpha"], con["beta"]))
         return out
     finally:
         ctx.close()
@@ -85,9 +87,18 @@ def test_skipped_evaluations_change_nothing(monkeypatch, request, kind, blocks, 
              b[0]["sweeps"], "fp32", a[0]["sweeps_fp32"], b[0]["sweeps_fp32"], "error", a[0]["error"], b[0]["error"])
         for key in ("ncontacts", "ncolours", "sweeps", "sweeps_fp32", "error", "succeeded"):
             assert a[0][key] == b[0][key], key
-        for x, y in zip(a[1:], b[1:]):
+        for x, y in zip(a[2:], b[2:]):
             if x is not None:
                 assert np.array_equal(x, y)
+        st, ev_ref, ev = a[0], a[1], b[1]
+        _say("evaluations", ev_ref, ev)
+        if st["ncontacts"] > 0:
+            s64 = st["sweeps"] - st["sweeps_fp32"]
+            assert ev_ref["skipped"] == 0
+            assert ev_ref["full_fp32"] == 1 + st["sweeps_fp32"]
+            assert ev_ref["full"] - ev_ref["full_fp32"] == 1 + s64
+            assert ev["full_fp32"] + ev["skipped"] == ev_ref["full_fp32"]
+            assert ev["full"] - ev["full_fp32"] == 1 + s64
         long_fp32 = long_fp32 or a[0]["sweeps_fp32"] > 1024
     if warm == scx.WARM_NONE:
         assert long_fp32  # a stagnation test ran
