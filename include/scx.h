@@ -282,14 +282,20 @@ int scx_unconstrained_flow(scx_ctx* ctx, int32_t integrator, uint32_t n, const d
 int scx_set_timing(scx_ctx* ctx, int32_t enable);
 int scx_get_timing(scx_ctx* ctx, uint32_t cap, const char** names, double* ms, uint32_t* count);
 
-/* Counters of the last step that returned stats: residual evaluations that were full passes
- * over the contacts (all phases, and of those the FP32 phase's), FP32 evaluations replaced
- * by the check of the contacts the last full one listed as near its maximum, and sweep
- * launches that swept their late colours as components. No counterpart in the reference
- * (which evaluates after every eval_every-th sweep); they give the sweep kernel's byte count
- * and show which paths a step took. Any pointer may be NULL. */
-int scx_get_sweep_counters(const scx_ctx* ctx, uint32_t* full_evals, uint32_t* full_evals_fp32,
-                           uint32_t* skipped_evals, uint32_t* component_launches);
+/* Counters of the sweeps of the last step that returned stats. No counterpart in the
+ * reference (which evaluates after every eval_every-th sweep and sweeps colour by colour);
+ * they give the sweep kernel's byte count and show which paths a step took. */
+typedef struct scx_sweep_counters {
+  uint32_t full_evals;          /* residual evaluations that were full passes over the contacts */
+  uint32_t full_evals_fp32;     /* of which in the FP32 phase */
+  uint32_t skipped_evals;       /* FP32 evaluations replaced by the check of the contacts the last
+                                   full one listed as near its maximum */
+  uint32_t component_launches;  /* sweep launches that swept their late colours as components */
+  uint32_t component_overflows; /* launches whose components overflowed a workgroup's list */
+  uint32_t component_width;     /* most contacts of one colour in a workgroup's list */
+  double component_share;       /* percent of the contacts the next step's component phase may hold */
+} scx_sweep_counters;
+int scx_get_sweep_counters(const scx_ctx* ctx, scx_sweep_counters* out);
 
 /* Online autotuning of the persistent Gauss-Seidel launches (no counterpart in the
  * reference, whose So-bogus solve has no launch shape; this is for callers that step

@@ -74,6 +74,13 @@ class ScxStepStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ScxSweepCounters(ctypes.Structure):
+    _fields_ = [("full_evals", ctypes.c_uint32), ("full_evals_fp32", ctypes.c_uint32),
+                ("skipped_evals", ctypes.c_uint32), ("component_launches", ctypes.c_uint32),
+                ("component_overflows", ctypes.c_uint32), ("component_width", ctypes.c_uint32),
+                ("component_share", _d)]
+
+
 class ScxContactSet(ctypes.Structure):
     _fields_ = [("ncontacts", ctypes.c_uint32), ("body0", _pi32), ("body1", _pi32), ("bases", _pd), ("arm0", _pd),
                 ("arm1", _pd), ("mu", _pd), ("CoR", _pd), ("nrel_extra", _pd), ("drel_extra", _pd)]
@@ -118,7 +125,7 @@ def lib():
         L.scx_unconstrained_flow.argtypes = [_vp, ctypes.c_int32, ctypes.c_uint32, _pd, _pd, _d, _pd, _pd, _pd, _pd]
         L.scx_set_timing.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_timing.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p), _pd, _pu32]
-        L.scx_get_sweep_counters.argtypes = [_vp, _pu32, _pu32, _pu32, _pu32]
+        L.scx_get_sweep_counters.argtypes = [_vp, ctypes.POINTER(ScxSweepCounters)]
         L.scx_set_autotune.argtypes = [_vp, ctypes.c_int32]
         L.scx_get_autotune.argtypes = [_vp, ctypes.c_uint32, _pu32, _pi32, _pi32, _pf, _pi32, _pu32]
         L.scx_set_warm_model.argtypes = [_vp, _pf, ctypes.c_uint32]
@@ -360,12 +367,16 @@ class Context:
         return [(names[i].decode(), float(ms[i])) for i in range(min(cnt.value, 32))]
 
     def sweep_counters(self):
-        """Counters of the last step: full residual evaluations (all phases and the FP32
-        phase's), FP32 evaluations replaced by the check of the listed contacts, and sweep
-        launches that ran the component phase."""
-        x = [ctypes.c_uint32(0) for _ in range(4)]
-        self._check(lib().scx_get_sweep_counters(self._h, *[ctypes.byref(v) for v in x]), "scx_get_sweep_counters")
-        return dict(zip(("full", "full_fp32", "skipped", "component_launches"), (v.value for v in x)))
+        """Counters of the last step's sweeps (scx_sweep_counters): full residual evaluations
+        (all phases and the FP32 phase's), FP32 evaluations replaced by the check of the
+        listed contacts, sweep launches that ran the component phase or overflowed it, the
+        widest colour of a workgroup's component list, and the share of the contacts the next
+        step's component phase may hold."""
+        x = ScxSweepCounters()
+        self._check(lib().scx_get_sweep_counters(self._h, ctypes.byref(x)), "scx_get_sweep_counters")
+        return {"full": x.full_evals, "full_fp32": x.full_evals_fp32, "skipped": x.skipped_evals,
+                "component_launches": x.component_launches, "component_overflows": x.component_overflows,
+                "component_width": x.component_width, "component_share": x.component_share}
 
     def set_autotune(self, on):
         """Online autotuning of the sweep launch width (results do not depend on it)."""

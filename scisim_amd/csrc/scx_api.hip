@@ -99,6 +99,14 @@ uint32_t next_pow2(uint64_t v) {
   return (uint32_t)std::min<uint64_t>(p, 1ull << 31);
 }
 
+// After a step with stats: the share of the contacts in the component phase for the next
+// step (one scene; comp_share_update).
+void comp_adapt_share(scx_ctx* c, const StepCounters& h) {
+  if (!c->comp_adapt || !c->comp_sweeps || c->nscenes || h.sweeps == 0) return;
+  comp_share_update(c->comp_share, c->comp_ceiling, h.comp_overflows, h.comp_launches, h.comp_maxw,
+                    (uint32_t)kGSBlock);
+}
+
 // Body-sized buffers.
 int alloc_bodies(scx_ctx* c, uint32_t N) {
   const int D = c->dim;
@@ -465,6 +473,7 @@ int step_once(scx_ctx* c, const scx_step_params* p, scx_step_stats* st, bool wan
     HIPC(hipMemcpyAsync(d.ctr_host, d.ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     c->last_ncolours = d.ctr_host->ncolours;
+    comp_adapt_share(c, *d.ctr_host);
     if (c->violator_skip == 2 && p->precision == SCX_PRECISION_FP32_POLISH && c->nscenes == 0 &&
         d.ctr_host->sweeps_fp32 >= 64 && d.ctr_host->evals_skipped == 0)
       return fail(c, SCX_E_STATE, "no FP32 evaluation skipped in %u sweeps", d.ctr_host->sweeps_fp32);
@@ -596,9 +605,12 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     // a one-workgroup tail instead of as components (the same iterates bit for bit; for A/B
     // measurements and tests)
     if (const char* env = getenv("SCX_COMPONENT_SWEEPS")) c->comp_sweeps = atoi(env) != 0;
-    // SCX_COMPONENT_PERCENT: the share of the contacts (in percent, 1 to 50) in the colours
-    // of the component phase at most (default 15)
-    if (const char* env = getenv("SCX_COMPONENT_PERCENT")) c->comp_percent = std::min(50, std::max(1, atoi(env)));
+    // SCX_COMPONENT_PERCENT: a fixed share of the contacts (in percent, 1 to 100) in the
+    // colours of the component phase at most (default: 20, then adapted step by step)
+    if (const char* env = getenv("SCX_COMPONENT_PERCENT")) {
+      c->comp_share = std::min(100.0, std::max(1.0, atof(env)));
+      c->comp_adapt = false;
+    }
     if ((e = dalloc(&d.partial, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
         (e = dalloc(&d.viol, (size_t)3 + kViolCap)) != hipSuccess ||
         (e = dalloc(&d.comp_cnt, (size_t)std::max(c->gs_blocks, 1) + 1)) != hipSuccess ||
@@ -1597,15 +1609,20 @@ int scx_get_timing(scx_ctx* c, uint32_t cap, const char** names, double* ms, uin
   return SCX_OK;
 }
 
-int scx_get_sweep_counters(const scx_ctx* c, uint32_t* full_evals, uint32_t* full_evals_fp32,
-                           uint32_t* skipped_evals, uint32_t* component_launches) {
-  if (!c) return SCX_E_ARG;
+int scx_get_sweep_counters(const scx_ctx* c, scx_sweep_counters* out) {
+  if (!c || !out) return SCX_E_ARG;
   // the step's counters, copied to the host at the end of a step that returned stats
   const StepCounters* h = c->d.ctr_host;
-  if (full_evals) *full_evals = h ? h->evals : 0u;
-  if (full_evals_fp32) *full_evals_fp32 = h ? h->evals_fp32 : 0u;
-  if (skipped_evals) *skipped_evals = h ? h->evals_skipped : 0u;
-  if (component_launches) *component_launches = h ? h->comp_launches : 0u;
+  *out = scx_sweep_counters{};
+  if (h) {
+    out->full_evals = h->evals;
+    out->full_evals_fp32 = h->evals_fp32;
+    out->skipped_evals = h->evals_skipped;
+    out->component_launches = h->comp_launches;
+    out->component_overflows = h->comp_overflows;
+    out->component_width = h->comp_maxw;
+  }
+  out->component_share = c->comp_share;
   return SCX_OK;
 }
 

@@ -21,6 +21,7 @@ constexpr int kMaxColours = 64;
 // Contacts listed per full residual evaluation of an FP32 phase (one per thread of the
 // sweep kernel's workgroup 0).
 constexpr uint32_t kViolCap = 512;
+constexpr int kGSBlock = 512;  // persistent sweep workgroup: 8 wave64s
 // Contacts of the component phase one workgroup of the persistent sweep kernel holds in LDS.
 constexpr uint32_t kCompCap = 2048;
 constexpr uint32_t kMaxScenePlanes = 64;        // static planes of one scene (64-bit plane masks)
@@ -66,6 +67,8 @@ struct StepCounters {
   uint32_t comp_launches;        // sweep launches that ran the component phase
   uint32_t evals;                // full residual evaluations (passes over the contacts), all phases
   uint32_t evals_fp32;           // of which in the FP32 phase
+  uint32_t comp_overflows;       // sweep launches whose components overflowed a workgroup's list
+  uint32_t comp_maxw;            // largest number of one colour's contacts a workgroup swept in the phase
   uint32_t pad;
   double error;                  // best residual of the last phase
   double error_fp32;
@@ -268,7 +271,13 @@ struct scx_ctx {
   bool coherent_sweeps = true;  // colour passes with coherent body accesses and light barriers
   int violator_skip = 1;        // FP32 phase: skip full evaluations while a listed violator remains (2: required)
   bool comp_sweeps = true;      // one scene: sweep the late colours as components, one workgroup each
-  int comp_percent = 15;        // the component phase starts where at most this share of the contacts remains
+  // The component phase starts where at most comp_share percent of the contacts remain.
+  // Adapted after every step with stats (comp_adapt): lowered when the phase overflowed a
+  // workgroup's list or one colour of a workgroup took more than one pass of its threads,
+  // raised while those stay well below one pass. SCX_COMPONENT_PERCENT fixes it.
+  double comp_share = 20.0;
+  double comp_ceiling = 100.0;  // below the last share that overflowed (drifts back up)
+  bool comp_adapt = true;
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,
   // hsa/hsa.h: "Queues of this type may be limited in number"); while another process holds
   // them a cooperative launch fails with hipErrorIllegalState. The persistent kernels need

@@ -29,7 +29,6 @@
 
 namespace scx {
 
-constexpr int kGSBlock = 512;  // persistent sweep workgroup: 8 wave64s
 
 template <typename Real>
 struct Vec4 {
@@ -792,7 +791,10 @@ __device__ uint32_t component_setup(const GSArgs<Real>& A, const uint32_t* off, 
       __hip_atomic_store(A.comp_cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   grid_sync_coherent(A.bar);
-  if (__hip_atomic_load(A.comp_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return ncol;
+  if (__hip_atomic_load(A.comp_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+    if (gtid == 0) ++A.ctr->comp_overflows;
+    return ncol;
+  }
   // counting sort of this workgroup's contacts by colour
   const uint32_t n = __hip_atomic_load(A.comp_cnt + 1 + blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t* src = A.comp_list + (size_t)blockIdx.x * kCompCap;
@@ -825,6 +827,12 @@ __device__ uint32_t component_setup(const GSArgs<Real>& A, const uint32_t* off, 
   for (uint32_t j = threadIdx.x; j < n; j += kGSBlock) {
     const uint32_t k = __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_list[atomicAdd(s_pos + colour_of(k), 1u)] = k;
+  }
+  if (threadIdx.x == 0) {
+    // the widest colour of this workgroup's list (for the choice of the phase's colours)
+    uint32_t w = 0;
+    for (uint32_t c = first; c < ncol; ++c) w = max(w, s_coff[c + 1] - s_coff[c]);
+    atomicMax(&A.ctr->comp_maxw, w);
   }
   __syncthreads();
   return first;
@@ -1542,15 +1550,18 @@ static hipError_t coop_gs(scx_ctx* c, GSArgs<Real> A, int max_blocks, const uint
   // coherent colour passes unless switched off or the body records exceed the 32-bit
   // byte offsets of the buffer accesses
   const bool coh = c->coherent_sweeps && (uint64_t)A.nbodies * kBodyVecs<D> * 4 * sizeof(Real) < (1ull << 31);
-  // Component phase (one scene, coherent passes): the colours after which at most 15% of
-  // the contacts remain (scx_ctx::comp_percent). In the contact graphs of packings and piles (a body touches about
-  // six others) the contacts of those colours form many small components, of a few dozen
-  // contacts at most; the phase starts before the tail and spans two colours at least.
+  // Component phase (one scene, coherent passes): the colours after which at most
+  // comp_share percent of the contacts remain. In the contact graphs of packings and piles
+  // the late colours' contacts form many small components, of a few dozen contacts (on a
+  // settled 262k-sphere pile at 20%; at 25% they join into one too large for a workgroup);
+  // in sparser graphs (a 2D packing as it collapses) far more of the colours do. The share
+  // adapts from step to step (comp_adapt_share); the phase starts before the tail and spans
+  // two colours at least.
   int comp = ncol;
   uint32_t comp_contacts = 0;
   if (coh && A.comp_parent && !A.cscene) {
     int k = ncol;
-    while (k > 0 && (uint64_t)(col_off[ncol] - col_off[k - 1]) * 100 <= (uint64_t)A.C * c->comp_percent) --k;
+    while (k > 0 && (double)(col_off[ncol] - col_off[k - 1]) * 100.0 <= (double)A.C * c->comp_share) --k;
     if (k < tail && ncol - k >= 2) {
       comp = k;
       comp_contacts = col_off[ncol] - col_off[k];

@@ -143,4 +143,28 @@ class TuneTable {
   std::map<uint32_t, TuneEntry> table_;
 };
 
+// Share (percent) of the contacts in the component phase of the sweeps, for the next step,
+// from the last step's counters: launches whose components overflowed a workgroup's list,
+// launches that ran the phase, and the largest count of one colour's contacts in a
+// workgroup's list. On an overflow (the share reached the contact graph's giant component)
+// the share drops to a tenth below itself, which also becomes a ceiling that drifts back up
+// by 0.5% a step (about one retry in 20 steps while the graph stays the same); lowered by a
+// fifth when a workgroup had more contacts of one colour than threads (`threads`); raised
+// by a tenth, up to the ceiling, while every workgroup's widest colour stays within a
+// quarter of its threads or no launch ran the phase (too few colours qualified). Results do
+// not depend on the share, only the time of a sweep.
+inline void comp_share_update(double& share, double& ceiling, uint32_t overflows, uint32_t launches,
+                              uint32_t maxw, uint32_t threads) {
+  ceiling = ceiling * 1.005 < 100.0 ? ceiling * 1.005 : 100.0;
+  if (overflows > 0) {
+    ceiling = 0.9 * share > 1.0 ? 0.9 * share : 1.0;
+    share = ceiling;
+  } else if (maxw > threads) {
+    share = 0.8 * share > 1.0 ? 0.8 * share : 1.0;
+  } else if (launches == 0 || maxw <= threads / 4) {
+    const double up = 1.1 * share < ceiling ? 1.1 * share : ceiling;
+    if (up > share) share = up;
+  }
+}
+
 }  // namespace scx

@@ -101,6 +101,50 @@ int main() {
     CHECK(T.pick(31u | (1u << 8), 32, 1024) == 2);
     CHECK(T.pick(31u, 32, 1024) == 0);  // 2D: no settled class of its kind
   }
+  // share of the contacts in the component phase
+  {
+    // a pile: the suffix of colours 6.. holds 15.1% of the contacts, of 5.. 24.6%; from 5 on
+    // they form one component too large for a workgroup; colour 6 is 211 wide at most in a
+    // workgroup's list
+    auto pile = [](double s, uint32_t& ovf, uint32_t& maxw) {
+      ovf = s >= 24.6 ? 1u : 0u;
+      maxw = s >= 24.6 ? 0u : (s >= 15.1 ? 211u : 100u);
+    };
+    double share = 20.0, ceiling = 100.0;
+    for (int k = 0; k < 200; ++k) {
+      uint32_t ovf, maxw;
+      pile(share, ovf, maxw);
+      CHECK(ovf == 0);
+      scx::comp_share_update(share, ceiling, ovf, 2, maxw, 512);
+    }
+    CHECK(share == 20.0);
+    // a sparse graph: small components at any share
+    share = 20.0;
+    ceiling = 100.0;
+    for (int k = 0; k < 20; ++k) scx::comp_share_update(share, ceiling, 0, 2, 40, 512);
+    CHECK(share == 100.0);
+    // a giant component from 40% on: few retries, most steps between 36% and 40%
+    share = 20.0;
+    ceiling = 100.0;
+    int overflows = 0, high = 0;
+    for (int k = 0; k < 400; ++k) {
+      const uint32_t ovf = share >= 40.0 ? 1u : 0u;
+      overflows += (int)ovf;
+      high += (int)(share >= 36.0 && share < 40.0);
+      scx::comp_share_update(share, ceiling, ovf, 2, ovf ? 0u : 60u, 512);
+    }
+    CHECK(overflows >= 1 && overflows <= 25);
+    CHECK(high >= 350);
+    // a workgroup with more contacts of one colour than threads: lowered
+    share = 50.0;
+    ceiling = 100.0;
+    scx::comp_share_update(share, ceiling, 0, 2, 700, 512);
+    CHECK(share == 40.0);
+    // no launch ran the phase (too few colours qualified): raised
+    share = 20.0;
+    scx::comp_share_update(share, ceiling, 0, 0, 0, 512);
+    CHECK(share > 21.9 && share < 22.1);
+  }
   if (failures) {
     std::fprintf(stderr, "%d checks failed\n", failures);
     return 1;

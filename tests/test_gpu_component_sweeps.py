@@ -1,7 +1,7 @@
 """The component phase of the persistent sweep kernel.
 
 For one scene swept with coherent passes on several workgroups, the late colours (those
-after which at most 15% of the contacts remain) are grouped at the start of every sweep
+after which at most 20% of the contacts remain) are grouped at the start of every sweep
 launch into connected components through their bodies (a union-find over the bodies), and
 every component goes to one workgroup, which sweeps its contacts colour by colour through
 its L2 between an agent-scope acquire and release, with one grid barrier for the whole
@@ -13,7 +13,7 @@ the FP64 polish (whose skipped evaluations then follow a check after the phase),
 64 workgroups and on the width the cost model picks, with steps that converge and with a
 sweep limit that returns the best iterate. The sweep counters must show the phase in the
 steps of the one run and in none of the other, so that the comparison is not between two
-runs of the same path. The circle packings have five or six colours, whose last 15% of the
+runs of the same path. The circle packings have five or six colours, whose last 20% of the
 contacts often lie in one colour (then there is no phase): they run with the share raised to
 35% (SCX_COMPONENT_PERCENT), the pile with the default.
 """
@@ -48,7 +48,7 @@ def pile():
 
 def _run(monkeypatch, kind, s, mode, blocks, precision, steps, limit):
     monkeypatch.setenv("SCX_COMPONENT_SWEEPS", str(mode))
-    monkeypatch.setenv("SCX_COMPONENT_PERCENT", "35" if kind == "circles" else "15")
+    monkeypatch.setenv("SCX_COMPONENT_PERCENT", "35" if kind == "circles" else "20")
     if blocks:
         monkeypatch.setenv("SCX_FORCE_PERSISTENT_BLOCKS", str(blocks))
     else:

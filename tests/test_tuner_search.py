@@ -2,7 +2,11 @@
 simulated launches: it settles on the fastest width of a cost bowl, climbs down when
 wider is slower, ignores gains under its margin, settles after untimed short launches,
 respects the launch limit and starts new classes from the nearest settled class of the
-same kind (tests/cpp/tuner_tests.cpp)."""
+same kind; and the share of the contacts in the sweeps' component phase, on simulated
+contact graphs: it holds on a pile whose late colours are small components, climbs to all
+contacts on a sparse graph, retries a share that overflowed only now and then, and backs
+off from a workgroup with more contacts of one colour than threads
+(tests/cpp/tuner_tests.cpp)."""
 
 import os
 import subprocess
