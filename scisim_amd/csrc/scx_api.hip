@@ -605,6 +605,9 @@ int scx_create(const scx_config* cfg, scx_ctx** out) {
     // a one-workgroup tail instead of as components (the same iterates bit for bit; for A/B
     // measurements and tests)
     if (const char* env = getenv("SCX_COMPONENT_SWEEPS")) c->comp_sweeps = atoi(env) != 0;
+    // SCX_COMPONENT_LDS=0: every workgroup sweeps its components through its L2, none in LDS
+    // (the same iterates bit for bit; for A/B measurements and tests)
+    if (const char* env = getenv("SCX_COMPONENT_LDS")) c->comp_lds = atoi(env) != 0;
     // SCX_COMPONENT_PERCENT: a fixed share of the contacts (in percent, 1 to 100) in the
     // colours of the component phase at most (default: 20, then adapted step by step)
     if (const char* env = getenv("SCX_COMPONENT_PERCENT")) {

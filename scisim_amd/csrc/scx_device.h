@@ -102,8 +102,10 @@ __device__ __forceinline__ void wait_stores() {
 // with the workgroup count. Measured with 512-thread workgroups: 1.5 us at 8 workgroups,
 // 2.9 us at 64, 4.8 us at 128 (a two-level tree of counters with a fence on each side:
 // 3.4, 7.1 and 14.0 us).
-// Layout (u32 words): [0] generation, [32] arrival counter (separate 128-byte lines).
-constexpr uint32_t kBarrierWords = 64;
+// Layout (u32 words): [0] generation, [32] arrival counter (separate 128-byte lines); the
+// coherent barrier below: [64] 64-bit arrival count, [96] barriers completed, [128] exit
+// count.
+constexpr uint32_t kBarrierWords = 160;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
   // one workgroup: its waves share the CU's L1, a workgroup barrier orders their
@@ -143,39 +145,57 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
 // Split in two: the arrival (after which this workgroup may issue loads of data that no
 // workgroup writes before the barrier, overlapping their latency with the wait) and the
 // wait. Thread 0 keeps the ticket.
+// The arrivals go to a 64-bit count that only grows during a launch: at its i-th barrier
+// (from 0) a workgroup finds the count in [i G, (i + 1) G) (every workgroup has passed
+// barrier i - 1, not every one has arrived at barrier i), so the arrival that finds
+// (i + 1) G - 1 is the last one and sets the completed-barrier word to i + 1, which the
+// others poll (on a line of its own, apart from the arrivals). Nothing is reset between
+// barriers. Every workgroup of a launch that uses this barrier calls grid_exit_coherent
+// before it ends: the last one resets both words for the next launch.
 struct GridTicket {
-  uint32_t gen;
-  bool last;
+  uint32_t target;
 };
+
+__device__ __forceinline__ unsigned long long* grid_coherent_count(uint32_t* bar) {
+  return reinterpret_cast<unsigned long long*>(bar + 64);
+}
 
 __device__ __forceinline__ GridTicket grid_arrive_coherent(uint32_t* bar) {
   wait_stores();
   __syncthreads();
-  GridTicket t{0u, true};
+  GridTicket t{0u};
   if (gridDim.x > 1 && threadIdx.x == 0) {
-    uint32_t* gen = bar;
-    uint32_t* cnt = bar + 32;
-    t.gen = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    wait_stores();  // the generation is read before this workgroup's arrival can complete it
-    t.last = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
-    if (t.last) {
-      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      wait_stores();  // the counter is reset before any workgroup can leave and arrive again
-      __hip_atomic_store(gen, t.gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    const unsigned long long v =
+        __hip_atomic_fetch_add(grid_coherent_count(bar), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t.target = (uint32_t)(v / gridDim.x) + 1u;
+    if (v % gridDim.x == gridDim.x - 1) __hip_atomic_store(bar + 96, t.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return t;
 }
 
 __device__ __forceinline__ void grid_wait_coherent(uint32_t* bar, const GridTicket& t) {
-  if (gridDim.x > 1 && threadIdx.x == 0 && !t.last)
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == t.gen) __builtin_amdgcn_s_sleep(1);
+  if (gridDim.x > 1 && threadIdx.x == 0)
+    while (__hip_atomic_load(bar + 96, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t.target)
+      __builtin_amdgcn_s_sleep(1);
   // keeps the compiler from moving the loads that follow above the wait
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   __syncthreads();
 }
 
 __device__ __forceinline__ void grid_sync_coherent(uint32_t* bar) { grid_wait_coherent(bar, grid_arrive_coherent(bar)); }
+
+// The end of a launch that used grid_sync_coherent, in every workgroup: the last workgroup
+// to get here (all others have passed their last barrier) resets the arrival count and the
+// completed-barrier word.
+__device__ __forceinline__ void grid_exit_coherent(uint32_t* bar) {
+  if (gridDim.x > 1 && threadIdx.x == 0) {
+    if (__hip_atomic_fetch_add(bar + 128, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+      __hip_atomic_store(grid_coherent_count(bar), 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(bar + 96, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(bar + 128, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
 
 // Time (us) of one barrier interval of a persistent launch over `width` items with `b`
 // workgroups of `threads`, from timings on MI355X. One workgroup: ~0.2 us barrier, ~1.9 us

@@ -276,6 +276,7 @@ struct scx_ctx {
   // workgroup's list or one colour of a workgroup took more than one pass of its threads,
   // raised while those stay well below one pass. SCX_COMPONENT_PERCENT fixes it.
   double comp_share = 20.0;
+  bool comp_lds = true;         // component phase: hold a workgroup's components in LDS when they fit
   double comp_ceiling = 100.0;  // below the last share that overflowed (drifts back up)
   bool comp_adapt = true;
   // The device's cooperative queues are limited in number (HSA_QUEUE_TYPE_COOPERATIVE,

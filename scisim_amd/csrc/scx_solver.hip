@@ -46,6 +46,22 @@ struct Rec {
   Real la, lb, wn, wt, mu, f;
 };
 
+// The rest of the contact frame from n (and s in 3D): t = n x s (3D), t = (-n_y, n_x) (2D).
+template <int D, typename Real>
+__device__ __forceinline__ void frame_from_ns(Rec<D, Real>& R) {
+  if (D == 3) {
+    R.t[0] = R.n[1] * R.s[2] - R.n[2] * R.s[1];
+    R.t[1] = R.n[2] * R.s[0] - R.n[0] * R.s[2];
+    R.t[2] = R.n[0] * R.s[1] - R.n[1] * R.s[0];
+  } else {
+    R.t[0] = -R.n[1];
+    R.t[1] = R.n[0];
+    R.t[2] = 0;
+    R.s[0] = R.s[1] = R.s[2] = 0;
+    R.la = R.lb = 0;
+  }
+}
+
 template <int D, typename Real>
 __device__ __forceinline__ void load_rec(const ContactSoA<Real>& c, uint32_t k, Rec<D, Real>& R) {
   R.a = c.a[k];
@@ -55,18 +71,10 @@ __device__ __forceinline__ void load_rec(const ContactSoA<Real>& c, uint32_t k, 
   if (D == 3) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) R.s[d] = c.s[3 * (size_t)k + d];
-    R.t[0] = R.n[1] * R.s[2] - R.n[2] * R.s[1];
-    R.t[1] = R.n[2] * R.s[0] - R.n[0] * R.s[2];
-    R.t[2] = R.n[0] * R.s[1] - R.n[1] * R.s[0];
     R.la = c.la[k];
     R.lb = c.lb[k];
-  } else {
-    R.t[0] = -R.n[1];
-    R.t[1] = R.n[0];
-    R.t[2] = 0;
-    R.s[0] = R.s[1] = R.s[2] = 0;
-    R.la = R.lb = 0;
   }
+  frame_from_ns<D, Real>(R);
   R.wn = c.wn[k];
   R.wt = c.wt[k];
   R.mu = c.mu[k];
@@ -294,6 +302,7 @@ struct GSArgs {
   uint32_t* comp_cnt;
   uint32_t* comp_list;
   uint32_t comp_col;
+  int comp_lds;  // hold a workgroup's components in LDS when they fit (else through its L2)
 };
 
 
@@ -456,30 +465,23 @@ __device__ __forceinline__ void load_contact(const GSArgs<Real>& A, uint32_t k, 
   L.r[2] = ld_imp<Coh>(rp + 2);
 }
 
-// One contact of a colour pass from its loaded record. copy_best: first store the
-// contact's impulse as it was before this sweep into rbest (the previous sweep's iterate
-// was the best one so far).
-template <int D, typename Real, bool Coh = false>
-__device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const Loaded<D, Real>& L, bool init,
-                                       bool copy_best, const BodyPort<Real>& P = BodyPort<Real>{}) {
+// One contact of a colour pass from its loaded record, on body vectors ia and ib (3D: ia + 1
+// and ib + 1 are the angular halves of the same records) read and written through ld/st,
+// the new impulse stored through st_r(d, x). copy_best: first store the contact's impulse
+// as it was before this sweep into rbest (the previous sweep's iterate was the best one so
+// far). Every update of a contact goes through this code, wherever its bodies are held.
+template <int D, typename Real, typename Ld, typename St, typename StR>
+__device__ __forceinline__ void update_core(const GSArgs<Real>& A, uint32_t k, const Loaded<D, Real>& L, bool init,
+                                            bool copy_best, size_t ia, size_t ib, Ld&& ld, St&& st, StR&& st_r) {
   using V = typename Vec4<Real>::type;
   const Rec<D, Real>& R = L.R;
   const bool hasb = R.b != kNoBody;
-  V* bv = reinterpret_cast<V*>(A.body);
-  const size_t ia = (size_t)kBodyVecs<D> * R.a, ib = (size_t)kBodyVecs<D> * R.b;
-  // body vector i (3D: i + 1 is the angular half of the same record)
-  auto ld = [&](size_t i) { return Coh ? P.ld(i) : bv[i]; };
-  auto st = [&](size_t i, const V& x) {
-    if (Coh) P.st(i, x);
-    else bv[i] = x;
-  };
   V va = ld(ia), wa = {0, 0, 0, 0}, vb = {0, 0, 0, 0}, wb = {0, 0, 0, 0};
   if (D == 3) wa = ld(ia + 1);
   if (hasb) {
     vb = ld(ib);
     if (D == 3) wb = ld(ib + 1);
   }
-  Real* rp = A.c.r + 3 * (size_t)k;
   const Real r[3] = {L.r[0], L.r[1], L.r[2]};
   if (copy_best) {
     Real* rb = A.rbest + 3 * (size_t)k;
@@ -499,9 +501,9 @@ __device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const 
     dr[0] = rn[0] - r[0];
     dr[1] = rn[1] - r[1];
     dr[2] = rn[2] - r[2];
-    st_imp<Coh>(rp, rn[0]);
-    st_imp<Coh>(rp + 1, rn[1]);
-    if (D == 3) st_imp<Coh>(rp + 2, rn[2]);
+    st_r(0, rn[0]);
+    st_r(1, rn[1]);
+    if (D == 3) st_r(2, rn[2]);
   }
   Real Pi[3];  // the impulse increment in world coordinates
 #pragma unroll
@@ -533,6 +535,23 @@ __device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const 
       st(ib + 1, wb);
     }
   }
+}
+
+// One contact of a colour pass on the body records in global memory (Coh: coherently).
+template <int D, typename Real, bool Coh = false>
+__device__ __forceinline__ void update(const GSArgs<Real>& A, uint32_t k, const Loaded<D, Real>& L, bool init,
+                                       bool copy_best, const BodyPort<Real>& P = BodyPort<Real>{}) {
+  using V = typename Vec4<Real>::type;
+  V* bv = reinterpret_cast<V*>(A.body);
+  Real* rp = A.c.r + 3 * (size_t)k;
+  update_core<D, Real>(
+      A, k, L, init, copy_best, (size_t)kBodyVecs<D> * L.R.a, (size_t)kBodyVecs<D> * L.R.b,
+      [&](size_t i) { return Coh ? P.ld(i) : bv[i]; },
+      [&](size_t i, const V& x) {
+        if (Coh) P.st(i, x);
+        else bv[i] = x;
+      },
+      [&](int d, Real x) { st_imp<Coh>(rp + d, x); });
 }
 
 // One contact of a colour pass (record loaded here).
@@ -838,6 +857,184 @@ __device__ uint32_t component_setup(const GSArgs<Real>& A, const uint32_t* off, 
   return first;
 }
 
+// A workgroup's components held in LDS for the whole launch when its list has at most
+// kCompLds contacts touching at most kCompLds bodies: the contact records and impulses of
+// its contacts (only this workgroup updates those impulses; every new value is also stored
+// coherently for the evaluations) and the body records, loaded from memory at the start of
+// the phase of every sweep and stored back at its end, both coherently. A colour of the
+// phase is then LDS traffic and a workgroup barrier. FP32 sweeps hold 1024 contacts, the
+// FP64 polish 512 (the same LDS bytes).
+template <typename Real>
+constexpr uint32_t kCompLds = (sizeof(Real) == 4) ? 1024u : 512u;
+template <int D>
+constexpr int kRecWords = (D == 3) ? 12 : 7;  // n (3), s (3), la, lb (3D), wn, wt, mu, f
+constexpr uint16_t kNoLocal = 0xFFFF;
+
+template <int D, typename Real>
+struct CompLds {
+  using V = typename Vec4<Real>::type;
+  V* body;          // kCompLds x kBodyVecs: the records of the bodies held
+  Real* r;          // 3 per listed contact: its impulse
+  Real* rec;        // kRecWords per listed contact: its record
+  uint16_t* eidx;   // 2 per listed contact: the LDS body of each side (kNoLocal: a plane)
+  uint32_t* slot;   // per LDS body: its slot
+  uint32_t* nb;     // bodies held; 0: the lists are swept through the L2 instead
+};
+
+template <int D, typename Real>
+__device__ __forceinline__ CompLds<D, Real> comp_lds_arrays() {
+  using V = typename Vec4<Real>::type;
+  constexpr uint32_t cap = kCompLds<Real>;
+  __shared__ V s_body[cap * kBodyVecs<D>];
+  __shared__ Real s_r[3 * cap];
+  __shared__ Real s_rec[kRecWords<D> * cap];
+  __shared__ uint16_t s_eidx[2 * cap];
+  __shared__ uint32_t s_slot[cap];
+  __shared__ uint32_t s_nb;
+  return CompLds<D, Real>{s_body, s_r, s_rec, s_eidx, s_slot, &s_nb};
+}
+
+// After component_setup (its last grid barrier ends every read of the union-find parents):
+// numbers the bodies of this workgroup's n listed contacts in LDS. A body of a component
+// belongs to that component's workgroup alone, so the workgroup reuses the body's parent
+// word as a claim: every endpoint clears it, the first endpoint to swap in its own number
+// owns the body and takes the next LDS slot, the others read the owner's slot. Loads the
+// listed contacts' records and impulses. Leaves *L.nb = 0 when the list or its bodies exceed
+// kCompLds.
+template <int D, typename Real>
+__device__ void comp_lds_setup(const GSArgs<Real>& A, const uint32_t* s_list, uint32_t n, const CompLds<D, Real>& L) {
+  uint32_t* par = A.comp_parent;
+  __shared__ uint32_t s_fail;
+  if (threadIdx.x == 0) {
+    *L.nb = 0;
+    s_fail = (n > kCompLds<Real> || !A.comp_lds) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_fail) {
+    __syncthreads();
+    return;
+  }
+  auto body_of = [&](uint32_t e) {
+    const uint32_t k = s_list[e >> 1];
+    return (e & 1) ? A.c.b[k] : A.c.a[k];
+  };
+  for (uint32_t e = threadIdx.x; e < 2 * n; e += kGSBlock) {
+    const uint32_t x = body_of(e);
+    if (x != kNoBody) __hip_atomic_store(par + x, kNoBody, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  wait_stores();
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < 2 * n; e += kGSBlock) {
+    const uint32_t x = body_of(e);
+    uint32_t expect = kNoBody;
+    if (x != kNoBody)
+      __hip_atomic_compare_exchange_strong(par + x, &expect, e, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+  }
+  wait_stores();
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < 2 * n; e += kGSBlock) {
+    const uint32_t x = body_of(e);
+    L.eidx[e] = kNoLocal;
+    if (x != kNoBody && __hip_atomic_load(par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == e) {
+      const uint32_t i = atomicAdd(L.nb, 1u);
+      if (i < kCompLds<Real>) {
+        L.eidx[e] = (uint16_t)i;
+        L.slot[i] = x;
+      } else {
+        s_fail = 1u;
+      }
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < 2 * n; e += kGSBlock) {
+    const uint32_t x = body_of(e);
+    if (x == kNoBody) continue;
+    const uint32_t owner = __hip_atomic_load(par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (owner != e) L.eidx[e] = L.eidx[owner];
+  }
+  for (uint32_t j = threadIdx.x; j < n; j += kGSBlock) {
+    const uint32_t k = s_list[j];
+    const Real* rp = A.c.r + 3 * (size_t)k;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) L.r[3 * j + d] = ld_imp<true>(rp + d);
+    Rec<D, Real> R;
+    load_rec<D, Real>(A.c, k, R);
+    Real* q = L.rec + (size_t)kRecWords<D> * j;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) q[d] = R.n[d];
+    if (D == 3) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[3 + d] = R.s[d];
+      q[6] = R.la;
+      q[7] = R.lb;
+    }
+    Real* w = q + kRecWords<D> - 4;
+    w[0] = R.wn;
+    w[1] = R.wt;
+    w[2] = R.mu;
+    w[3] = R.f;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_fail) *L.nb = 0;
+  __syncthreads();
+}
+
+// One sweep of this workgroup's components held in LDS (comp_lds_setup): the body records in,
+// the colours [ccol, ncol) of its list with workgroup barriers, the records out.
+template <int D, typename Real>
+__device__ void comp_lds_sweep(const GSArgs<Real>& A, const uint32_t* s_list, const uint32_t* s_coff, uint32_t ccol,
+                               uint32_t ncol, const CompLds<D, Real>& L, const BodyPort<Real>& P, bool copy) {
+  using V = typename Vec4<Real>::type;
+  const uint32_t nv = *L.nb * kBodyVecs<D>;
+  for (uint32_t i = threadIdx.x; i < nv; i += kGSBlock)
+    L.body[i] = P.ld((size_t)kBodyVecs<D> * L.slot[i / kBodyVecs<D>] + i % kBodyVecs<D>);
+  __syncthreads();
+  for (uint32_t col = ccol; col < ncol; ++col) {
+    const uint32_t lo = s_coff[col], hi = s_coff[col + 1];
+    if (lo == hi) continue;  // the same in every thread of the workgroup
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += kGSBlock) {
+      const uint32_t k = s_list[j];
+      const uint16_t ea = L.eidx[2 * j], eb = L.eidx[2 * j + 1];
+      Loaded<D, Real> ld;
+      // the record as load_rec builds it (a and b: only whether b is a body matters here)
+      Rec<D, Real>& R = ld.R;
+      const Real* q = L.rec + (size_t)kRecWords<D> * j;
+      R.a = 0;
+      R.b = (eb == kNoLocal) ? kNoBody : 0u;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) R.n[d] = q[d];
+      if (D == 3) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) R.s[d] = q[3 + d];
+        R.la = q[6];
+        R.lb = q[7];
+      }
+      frame_from_ns<D, Real>(R);
+      const Real* w = q + kRecWords<D> - 4;
+      R.wn = w[0];
+      R.wt = w[1];
+      R.mu = w[2];
+      R.f = w[3];
+      Real* rl = L.r + 3 * j;
+      ld.r[0] = rl[0];
+      ld.r[1] = rl[1];
+      ld.r[2] = rl[2];
+      Real* rp = A.c.r + 3 * (size_t)k;
+      update_core<D, Real>(
+          A, k, ld, false, copy, (size_t)kBodyVecs<D> * ea, (size_t)kBodyVecs<D> * (eb == kNoLocal ? 0u : eb),
+          [&](size_t i) { return L.body[i]; }, [&](size_t i, const V& x) { L.body[i] = x; },
+          [&](int d, Real x) {
+            rl[d] = x;
+            st_imp<true>(rp + d, x);
+          });
+    }
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < nv; i += kGSBlock)
+    P.st((size_t)kBodyVecs<D> * L.slot[i / kBodyVecs<D>] + i % kBodyVecs<D>, L.body[i]);
+}
+
 // FP32 phase, workgroup 0: is one of the contacts listed by the last full evaluation (one
 // per thread) still at or above `stop`? The answer goes to A.vflag for every workgroup
 // after the next grid barrier.
@@ -904,8 +1101,16 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
   __shared__ uint32_t s_clist[kCompCap];
   __shared__ uint32_t s_coff[kMaxColours + 1];
   uint32_t ccol = ncol;
-  if (Coh && A.comp_parent && !A.cscene && gridDim.x > 1 && A.comp_col < ncol)
-    ccol = component_setup<D, Real>(A, off, ncol, A.comp_col, gtid, gsize, s_clist, s_coff);
+  CompLds<D, Real> CL{};
+  if constexpr (Coh) {
+    if (A.comp_parent && !A.cscene && gridDim.x > 1 && A.comp_col < ncol) {
+      ccol = component_setup<D, Real>(A, off, ncol, A.comp_col, gtid, gsize, s_clist, s_coff);
+      if (ccol < ncol) {
+        CL = comp_lds_arrays<D, Real>();
+        comp_lds_setup<D, Real>(A, s_clist, s_coff[ncol], CL);
+      }
+    }
+  }
   if (ccol < ncol && gtid == 0) ++A.ctr->comp_launches;
   // FP32 phase of one scene with a one-workgroup tail: after a full evaluation that listed
   // the contacts near its maximum, a sweep's full evaluation is skipped while workgroup 0
@@ -980,18 +1185,23 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         }
       }
       if (ccol < ncol) {
-        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        __syncthreads();
-        for (uint32_t col = ccol; col < ncol; ++col) {
-          const uint32_t lo = s_coff[col], hi = s_coff[col + 1];
-          if (lo == hi) continue;  // the same in every thread of the workgroup
-          for (uint32_t j = lo + threadIdx.x; j < hi; j += kGSBlock) process<D, Real>(A, s_clist[j], false, copy);
+        if (Coh && *CL.nb > 0) {
+          if constexpr (Coh) comp_lds_sweep<D, Real>(A, s_clist, s_coff, ccol, ncol, CL, P, copy);
+        } else {
+          // through the L2 between an agent-scope acquire and release
+          if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          __syncthreads();
+          for (uint32_t col = ccol; col < ncol; ++col) {
+            const uint32_t lo = s_coff[col], hi = s_coff[col + 1];
+            if (lo == hi) continue;  // the same in every thread of the workgroup
+            for (uint32_t j = lo + threadIdx.x; j < hi; j += kGSBlock) process<D, Real>(A, s_clist[j], false, copy);
+            __syncthreads();
+          }
+          wait_stores();
+          __syncthreads();
+          if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
           __syncthreads();
         }
-        wait_stores();
-        __syncthreads();
-        if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
         grid_sync_coherent(A.bar);
         if (may_skip) {
           // the listed contacts may lie in any workgroup's components: checked after the
@@ -1105,6 +1315,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
     A.ctr->error = best;
     A.ctr->evals += ev;
   }
+  if (Coh) grid_exit_coherent(A.bar);
 }
 
 // ---------------------------------------------------------------------------
@@ -1675,6 +1886,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
       A.comp_parent = c->d.comp_parent;
       A.comp_cnt = c->d.comp_cnt;
       A.comp_list = c->d.comp_list;
+      A.comp_lds = c->comp_lds ? 1 : 0;
     }
     if (ens) {
       A.cscene = c->d.cscene;
@@ -1703,6 +1915,7 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
     B.comp_parent = c->d.comp_parent;
     B.comp_cnt = c->d.comp_cnt;
     B.comp_list = c->d.comp_list;
+    B.comp_lds = c->comp_lds ? 1 : 0;
   }
   if (ens) {
     B.cscene = c->d.cscene;

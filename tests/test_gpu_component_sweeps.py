@@ -11,7 +11,10 @@ so the iterates must be those of the grid-wide passes and the one-workgroup tail
 residuals, on a 2D circle packing and a 3D sphere pile, in FP64 and in FP32 sweeps with
 the FP64 polish (whose skipped evaluations then follow a check after the phase), on 16 and
 64 workgroups and on the width the cost model picks, with steps that converge and with a
-sweep limit that returns the best iterate. The sweep counters must show the phase in the
+sweep limit that returns the best iterate; with each workgroup's components held in LDS for
+the launch (bodies loaded and stored coherently around the phase, impulses kept in LDS and
+written through) and, with SCX_COMPONENT_LDS=0, swept through the L2 with ordinary accesses
+between an acquire and a release. The sweep counters must show the phase in the
 steps of the one run and in none of the other, so that the comparison is not between two
 runs of the same path. The circle packings have five or six colours, whose last 20% of the
 contacts often lie in one colour (then there is no phase): they run with the share raised to
@@ -46,8 +49,9 @@ def pile():
     return scenes.spheres_pile(n=8000, seed=13)
 
 
-def _run(monkeypatch, kind, s, mode, blocks, precision, steps, limit):
+def _run(monkeypatch, kind, s, mode, blocks, precision, steps, limit, lds=True):
     monkeypatch.setenv("SCX_COMPONENT_SWEEPS", str(mode))
+    monkeypatch.setenv("SCX_COMPONENT_LDS", "1" if lds else "0")
     monkeypatch.setenv("SCX_COMPONENT_PERCENT", "35" if kind == "circles" else "20")
     if blocks:
         monkeypatch.setenv("SCX_FORCE_PERSISTENT_BLOCKS", str(blocks))
@@ -74,15 +78,17 @@ def _run(monkeypatch, kind, s, mode, blocks, precision, steps, limit):
         ctx.close()
 
 
+@pytest.mark.parametrize("where", ["lds", "l2"])
 @pytest.mark.parametrize("limit", ["converged", "sweep_limit"])
 @pytest.mark.parametrize("precision", [scx.FP32_POLISH, scx.FP64], ids=["fp32_polish", "fp64"])
 @pytest.mark.parametrize("blocks", [16, 64, 0])
 @pytest.mark.parametrize("kind", ["circles", "pile"])
-def test_component_phase_matches_colour_passes_bit_for_bit(monkeypatch, request, kind, blocks, precision, limit):
+def test_component_phase_matches_colour_passes_bit_for_bit(monkeypatch, request, kind, blocks, precision, limit,
+                                                           where):
     s = request.getfixturevalue(kind)
-    _say("case", kind, "blocks", blocks, "precision", precision, "limit", limit)
+    _say("case", kind, "blocks", blocks, "precision", precision, "limit", limit, "where", where)
     ref = _run(monkeypatch, kind, s, 0, blocks, precision, 4, limit)
-    got = _run(monkeypatch, kind, s, 1, blocks, precision, 4, limit)
+    got = _run(monkeypatch, kind, s, 1, blocks, precision, 4, limit, lds=where == "lds")
     swept = limited = with_phase = 0
     for k, (a, b) in enumerate(zip(ref, got)):
         _say("step", k, "contacts", a[0]["ncontacts"], "colours", a[0]["ncolours"], "sweeps", a[0]["sweeps"],
