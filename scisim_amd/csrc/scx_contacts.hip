@@ -285,20 +285,44 @@ __global__ void __launch_bounds__(kBlock) k_adj_fill(uint32_t C, const uint32_t*
 // takes the smallest colour unused by its coloured neighbours. Neighbours coloured
 // during round k still count as uncoloured in round k, so decisions never depend
 // on scheduling. state = (round << 8) | colour, or -1.
+//
+// Each workgroup owns one contiguous range of contacts and keeps the ones still
+// uncoloured in an LDS list (when the range fits kColourLds), so a round visits only
+// those. Every access to state and to the round counts is an atomic that goes to memory,
+// so the rounds are separated by grid_sync_coherent alone, one per round: the contacts
+// coloured in round r are added to count r % 3, which every workgroup reads after the
+// barrier (the same value everywhere, hence the same exit), and count (r + 2) % 3, read a
+// round earlier, is cleared for round r + 2.
+constexpr uint32_t kColourLds = 4096;
+
 __global__ void __launch_bounds__(kBlock) k_colour(uint32_t C, const uint32_t* __restrict__ ca,
                                                    const uint32_t* __restrict__ cb, const uint32_t* __restrict__ off,
                                                    const uint32_t* __restrict__ adj, int32_t* state,
                                                    StepCounters* ctr, uint32_t* bar) {
-  const uint32_t gsize = gridDim.x * blockDim.x;
-  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  for (uint32_t k = gtid; k < C; k += gsize) state[k] = -1;
-  grid_barrier(bar);
-  uint32_t done = 0;
+  __shared__ uint16_t s_list[2][kColourLds];
+  __shared__ uint32_t s_n[2];
+  uint32_t* cnt = bar + 144;  // three round counts
+  const uint32_t per = (C + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = min(C, blockIdx.x * per), hi = min(C, lo + per);
+  const bool listed = per <= kColourLds;
+  for (uint32_t k = lo + threadIdx.x; k < hi; k += kBlock) {
+    __hip_atomic_store(&state[k], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (listed) s_list[0][k - lo] = (uint16_t)(k - lo);
+  }
+  if (threadIdx.x == 0) {
+    s_n[0] = hi - lo;
+    s_n[1] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 3) __hip_atomic_store(cnt + threadIdx.x, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  grid_sync_coherent(bar);
+  uint32_t done = 0, cur = 0;
   for (uint32_t round = 0; round <= C && done < C; ++round) {
     uint32_t mine = 0;
     uint32_t maxcol = 0;
-    for (uint32_t c = gtid; c < C; c += gsize) {
-      if (state[c] >= 0) continue;
+    const uint32_t nk = listed ? s_n[cur] : hi - lo;
+    for (uint32_t i = threadIdx.x; i < nk; i += kBlock) {
+      const uint32_t c = lo + (listed ? (uint32_t)s_list[cur][i] : i);
+      if (!listed && __hip_atomic_load(&state[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= 0) continue;
       const unsigned long long kc = prio_key(c);
       const uint32_t bodies[2] = {ca[c], cb[c]};
       bool is_max = true;
@@ -308,7 +332,7 @@ __global__ void __launch_bounds__(kBlock) k_colour(uint32_t C, const uint32_t* _
         for (uint32_t e = off[bodies[q]]; e < off[bodies[q] + 1]; ++e) {
           const uint32_t d = adj[e];
           if (d == c) continue;
-          const int32_t sd = __atomic_load_n(&state[d], __ATOMIC_RELAXED);
+          const int32_t sd = __hip_atomic_load(&state[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           const bool unc = (sd < 0) || ((uint32_t)sd >> 8) == round;
           if (unc) {
             if (prio_key(d) > kc) {
@@ -320,27 +344,35 @@ __global__ void __launch_bounds__(kBlock) k_colour(uint32_t C, const uint32_t* _
           }
         }
       }
-      if (!is_max) continue;
+      if (!is_max) {
+        if (listed) s_list[cur ^ 1][atomicAdd(&s_n[cur ^ 1], 1u)] = (uint16_t)(c - lo);
+        continue;
+      }
       const int col = (used == ~0ull) ? 64 : __ffsll((long long)~used) - 1;
       if (col >= kMaxColours) {
         atomicOr(&ctr->overflow, 8u);
-        __atomic_store_n(&state[c], (int32_t)((round << 8) | 63), __ATOMIC_RELAXED);
+        __hip_atomic_store(&state[c], (int32_t)((round << 8) | 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        __atomic_store_n(&state[c], (int32_t)((round << 8) | col), __ATOMIC_RELAXED);
+        __hip_atomic_store(&state[c], (int32_t)((round << 8) | col), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         maxcol = max(maxcol, (uint32_t)col + 1);
       }
       ++mine;
     }
     mine = block_reduce_sum_u32(mine);
-    if (threadIdx.x == 0 && mine) atomicAdd(&ctr->remaining, mine);
+    if (threadIdx.x == 0 && mine) atomicAdd(cnt + round % 3, mine);
     if (maxcol) atomicMax(&ctr->ncolours, maxcol);
-    grid_barrier(bar);
-    done = __atomic_load_n(&ctr->remaining, __ATOMIC_RELAXED);
-    if (gtid == 0 && done >= C) ctr->nrounds = round + 1;
-    // every workgroup has read the count before any adds to it again, so all of them
-    // take the same exit decision
-    grid_barrier(bar);
+    grid_sync_coherent(bar);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      __hip_atomic_store(cnt + (round + 2) % 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    done += __hip_atomic_load(cnt + round % 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && done >= C) ctr->nrounds = round + 1;
+    // the next round's list is complete (every thread passed the barrier); the old one is
+    // emptied for the round after
+    if (threadIdx.x == 0) s_n[cur] = 0;
+    cur ^= 1;
+    __syncthreads();
   }
+  grid_exit_coherent(bar);
 }
 
 // Colour masks for the stable colour-major ordering.

@@ -104,7 +104,7 @@ __device__ __forceinline__ void wait_stores() {
 // 3.4, 7.1 and 14.0 us).
 // Layout (u32 words): [0] generation, [32] arrival counter (separate 128-byte lines); the
 // coherent barrier below: [64] 64-bit arrival count, [96] barriers completed, [128] exit
-// count.
+// count; [144..146] free for a kernel's own counts.
 constexpr uint32_t kBarrierWords = 160;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
