@@ -675,6 +675,18 @@ __device__ double eval_residual(const GSArgs<Real>& A, uint32_t gtid, uint32_t g
 #pragma unroll 4
       for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, (double)contact_err<D, Real, float, Coh>(A, k, P));
     }
+  } else if (capture >= 0.0) {
+    // FP64: the same list
+    if (gtid == 0) __hip_atomic_store(A.viol + ((ev + 1) & 1), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll 4
+    for (uint32_t k = gtid; k < A.C; k += gsize) {
+      const double e = contact_err<D, Real, double, Coh>(A, k, P);
+      m = fmax(m, e);
+      if (e >= capture) {
+        const uint32_t i = atomicAdd(A.viol + (ev & 1), 1u);
+        if (i < kViolCap) __hip_atomic_store(A.viol + 2 + i, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   } else {
 #pragma unroll 4
     for (uint32_t k = gtid; k < A.C; k += gsize) m = fmax(m, contact_err<D, Real, double, Coh>(A, k, P));
@@ -1035,25 +1047,27 @@ __device__ void comp_lds_sweep(const GSArgs<Real>& A, const uint32_t* s_list, co
     P.st((size_t)kBodyVecs<D> * L.slot[i / kBodyVecs<D>] + i % kBodyVecs<D>, L.body[i]);
 }
 
-// FP32 phase, workgroup 0: is one of the contacts listed by the last full evaluation (one
-// per thread) still at or above `stop`? The answer goes to A.vflag for every workgroup
-// after the next grid barrier.
+// Workgroup 0: is one of the contacts listed by the last full evaluation (one per thread)
+// still at or above `th`? Each residual is the one the full evaluation computes (FP32
+// estimate in the FP32 phase, FP64 otherwise). The answer goes to A.vflag for every
+// workgroup after the next grid barrier.
 template <int D, typename Real, bool Coh>
-__device__ void violator_check(const GSArgs<Real>& A, uint32_t ev, double stop, const BodyPort<Real>& P,
+__device__ void violator_check(const GSArgs<Real>& A, uint32_t ev, double th, const BodyPort<Real>& P,
                                double* s_fm) {
   const uint32_t cnt =
       min(__hip_atomic_load(A.viol + ((ev - 1) & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), kViolCap);
   double e = 0.0;
-  if (threadIdx.x < cnt)
-    e = (double)contact_err<D, Real, float, Coh>(
-        A, __hip_atomic_load(A.viol + 2 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), P);
+  if (threadIdx.x < cnt) {
+    const uint32_t k = __hip_atomic_load(A.viol + 2 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    e = (A.phase == 1) ? (double)contact_err<D, Real, float, Coh>(A, k, P) : contact_err<D, Real, double, Coh>(A, k, P);
+  }
   e = wave_reduce_max_f64(e);
   if ((threadIdx.x & 63) == 0) s_fm[threadIdx.x >> 6] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
     double mx = s_fm[0];
     for (uint32_t w = 1; w < kGSBlock / 64; ++w) mx = fmax(mx, s_fm[w]);
-    __hip_atomic_store(A.vflag, (cnt > 0 && mx >= stop) ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(A.vflag, (cnt > 0 && mx >= th) ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
 }
@@ -1112,15 +1126,20 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
     }
   }
   if (ccol < ncol && gtid == 0) ++A.ctr->comp_launches;
-  // FP32 phase of one scene with a one-workgroup tail: after a full evaluation that listed
-  // the contacts near its maximum, a sweep's full evaluation is skipped while workgroup 0
-  // finds one of them still at or above `stop` (then the maximum is too, and the solve goes
-  // on exactly as the full evaluation would decide). Only in the first kSkipUntil sweeps
-  // and with a full evaluation at least every kFullEvery: the stagnation test from sweep
-  // 1024 on compares best residuals, which every sweep from kSkipUntil on contributes to as
-  // before (a warm-started step converges well within kSkipUntil sweeps).
+  // One scene with a one-workgroup tail or a component phase: after a full evaluation that
+  // listed the contacts near its maximum, a sweep's full evaluation is skipped while
+  // workgroup 0 finds one of them still at or above a threshold th, with a full evaluation
+  // at least every kFullEvery sweeps. th = the best residual so far: then the maximum is at
+  // least the best, so the full evaluation could only have said "go on, no new best" (the
+  // solve is not converged, as the best is at or above the stopping value while it runs,
+  // and neither the best iterate nor the best residual changes). The FP32 phase's
+  // stagnation test (every 1024 sweeps) compares best residuals, which skipped sweeps thus
+  // leave as they are; its sweeps are evaluated in full. In the first kSkipUntil sweeps of
+  // the FP32 phase, which keeps no best iterate, th = the stopping value (a warm-started
+  // step converges well within them, and the best residual by the first stagnation test at
+  // sweep 1024 is set by the sweeps after them).
   constexpr uint32_t kFullEvery = 256, kSkipUntil = 1024 - kFullEvery;
-  const bool subset = A.viol && A.phase == 1 && !A.cscene && (min(A.tail_col, ncol) < ncol || ccol < ncol);
+  const bool subset = A.viol && !A.cscene && (min(A.tail_col, ncol) < ncol || ccol < ncol);
   double err = eval_residual<D, Real, Coh>(A, gtid, gsize, ev++, stop, base, 0, nullptr, 0, P,
                                            subset ? stop : -1.0);
   uint32_t last_full = 0;
@@ -1156,7 +1175,9 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
       const uint32_t tail = min(A.tail_col, ncol);
       const uint32_t nwide = (ccol < ncol) ? ccol : tail;  // colours swept in grid-wide passes
       const bool eval_now = A.eval_every > 0 && it % A.eval_every == 0;
-      const bool may_skip = subset && eval_now && it < kSkipUntil && it - last_full < kFullEvery;
+      const bool stagnation_due = A.phase == 1 && it - ref_it >= 1024;
+      const bool may_skip = subset && eval_now && it - last_full < kFullEvery && !stagnation_due;
+      const double th = (A.phase == 1 && it < kSkipUntil) ? stop : fmax(stop, best);
       // the first contact of each colour pass is loaded before the barrier that opens it
       // completes (between the arrival and the wait of a coherent barrier), so that only the
       // body gathers wait for the barrier
@@ -1206,7 +1227,7 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
         if (may_skip) {
           // the listed contacts may lie in any workgroup's components: checked after the
           // barrier, and one more barrier publishes the answer
-          if (blockIdx.x == 0) violator_check<D, Real, Coh>(A, ev, stop, P, s_fm);
+          if (blockIdx.x == 0) violator_check<D, Real, Coh>(A, ev, th, P, s_fm);
           grid_sync_coherent(A.bar);
         }
       } else if (tail < ncol) {
@@ -1236,14 +1257,14 @@ __global__ void __launch_bounds__(kGSBlock) k_gs(GSArgs<Real> A) {
             if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __syncthreads();
           }
-          if (may_skip) violator_check<D, Real, Coh>(A, ev, stop, P, s_fm);
+          if (may_skip) violator_check<D, Real, Coh>(A, ev, th, P, s_fm);
         }
         if (Coh) grid_sync_coherent(A.bar);
         else grid_barrier(A.bar);
       }
       if (may_skip && __hip_atomic_load(A.vflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
         if (gtid == 0) ++A.ctr->evals_skipped;
-        continue;  // a listed contact is still at or above stop: not converged, no full evaluation
+        continue;  // a listed contact is still at or above th: the full evaluation would change nothing
       }
       if (eval_now) {
         last_full = it;
@@ -1911,6 +1932,12 @@ static hipError_t solve_dim(scx_ctx* c, uint32_t C, const uint32_t* col_off, con
                    c->d.partial, c->d.ctr, c->d.barrier, p->max_iters, p->eval_every, p->tol, fp32 ? 2 : 0,
                    (fp32 || warm) ? 1 : 0};
   B.lmap = c->d.lmap;
+  if (!ens && c->d.viol && c->violator_skip) {
+    e = hipMemsetAsync(c->d.viol, 0, sizeof(uint32_t) * (kViolCap + 3), s);
+    if (e != hipSuccess) return e;
+    B.viol = c->d.viol;
+    B.vflag = c->d.viol + 2 + kViolCap;
+  }
   if (!ens && c->comp_sweeps) {
     B.comp_parent = c->d.comp_parent;
     B.comp_cnt = c->d.comp_cnt;
