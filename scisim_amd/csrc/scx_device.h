@@ -103,9 +103,11 @@ __device__ __forceinline__ void wait_stores() {
 // 2.9 us at 64, 4.8 us at 128 (a two-level tree of counters with a fence on each side:
 // 3.4, 7.1 and 14.0 us).
 // Layout (u32 words): [0] generation, [32] arrival counter (separate 128-byte lines); the
-// coherent barrier below: [64] 64-bit arrival count, [96] barriers completed, [128] exit
-// count; [144..146] free for a kernel's own counts.
-constexpr uint32_t kBarrierWords = 160;
+// coherent barrier below: [64] 64-bit arrival count of the groups, [96] barriers
+// completed, [128] exit count, [160 + 32 g] 64-bit arrival count of group g (8 groups);
+// [144..146] free for a kernel's own counts.
+constexpr uint32_t kBarrierWords = 160 + 32 * 8;
+constexpr uint32_t kBarrierGroups = 8;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
   // one workgroup: its waves share the CU's L1, a workgroup barrier orders their
@@ -145,13 +147,16 @@ __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
 // Split in two: the arrival (after which this workgroup may issue loads of data that no
 // workgroup writes before the barrier, overlapping their latency with the wait) and the
 // wait. Thread 0 keeps the ticket.
-// The arrivals go to a 64-bit count that only grows during a launch: at its i-th barrier
-// (from 0) a workgroup finds the count in [i G, (i + 1) G) (every workgroup has passed
+// Two levels of arrival counts that only grow during a launch: workgroup w arrives in
+// group w mod 8 (the workgroups of one XCD, as the dispatcher deals them round-robin), and
+// the last arrival of each group arrives at the level above; so no count takes more than 32
+// of the 256 arrivals of a full chip. At its i-th barrier (from 0) a workgroup finds its
+// group's count in [i n, (i + 1) n) for the group's n workgroups (every workgroup has passed
 // barrier i - 1, not every one has arrived at barrier i), so the arrival that finds
-// (i + 1) G - 1 is the last one and sets the completed-barrier word to i + 1, which the
-// others poll (on a line of its own, apart from the arrivals). Nothing is reset between
-// barriers. Every workgroup of a launch that uses this barrier calls grid_exit_coherent
-// before it ends: the last one resets both words for the next launch.
+// (i + 1) n - 1 is its group's last; likewise for the groups above, whose last arrival sets
+// the completed-barrier word to i + 1, which the others poll (on a line of its own). Nothing
+// is reset between barriers. Every workgroup of a launch that uses this barrier calls
+// grid_exit_coherent before it ends: the last one resets the words for the next launch.
 struct GridTicket {
   uint32_t target;
 };
@@ -159,16 +164,27 @@ struct GridTicket {
 __device__ __forceinline__ unsigned long long* grid_coherent_count(uint32_t* bar) {
   return reinterpret_cast<unsigned long long*>(bar + 64);
 }
+__device__ __forceinline__ unsigned long long* grid_group_count(uint32_t* bar, uint32_t g) {
+  return reinterpret_cast<unsigned long long*>(bar + 160 + 32 * g);
+}
 
 __device__ __forceinline__ GridTicket grid_arrive_coherent(uint32_t* bar) {
   wait_stores();
   __syncthreads();
   GridTicket t{0u};
   if (gridDim.x > 1 && threadIdx.x == 0) {
+    const uint32_t G = gridDim.x;
+    const uint32_t ng = G < kBarrierGroups ? G : kBarrierGroups;
+    const uint32_t g = blockIdx.x % ng;
+    const uint32_t n = (G - g + ng - 1) / ng;  // workgroups g, g + ng, ... below G
     const unsigned long long v =
-        __hip_atomic_fetch_add(grid_coherent_count(bar), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t.target = (uint32_t)(v / gridDim.x) + 1u;
-    if (v % gridDim.x == gridDim.x - 1) __hip_atomic_store(bar + 96, t.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(grid_group_count(bar, g), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t.target = (uint32_t)(v / n) + 1u;
+    if (v % n == n - 1) {
+      const unsigned long long u =
+          __hip_atomic_fetch_add(grid_coherent_count(bar), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (u % ng == ng - 1) __hip_atomic_store(bar + 96, t.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   return t;
 }
@@ -190,6 +206,8 @@ __device__ __forceinline__ void grid_sync_coherent(uint32_t* bar) { grid_wait_co
 __device__ __forceinline__ void grid_exit_coherent(uint32_t* bar) {
   if (gridDim.x > 1 && threadIdx.x == 0) {
     if (__hip_atomic_fetch_add(bar + 128, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+      for (uint32_t g = 0; g < kBarrierGroups; ++g)
+        __hip_atomic_store(grid_group_count(bar, g), 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(grid_coherent_count(bar), 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(bar + 96, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(bar + 128, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
