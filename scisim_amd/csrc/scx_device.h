@@ -108,6 +108,7 @@ __device__ __forceinline__ void wait_stores() {
 // [144..146] free for a kernel's own counts.
 constexpr uint32_t kBarrierWords = 160 + 32 * 8;
 constexpr uint32_t kBarrierGroups = 8;
+constexpr uint32_t kBarrierTwoLevel = 128;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* bar) {
   // one workgroup: its waves share the CU's L1, a workgroup barrier orders their
@@ -174,16 +175,21 @@ __device__ __forceinline__ GridTicket grid_arrive_coherent(uint32_t* bar) {
   GridTicket t{0u};
   if (gridDim.x > 1 && threadIdx.x == 0) {
     const uint32_t G = gridDim.x;
-    const uint32_t ng = G < kBarrierGroups ? G : kBarrierGroups;
+    // below kBarrierTwoLevel workgroups one group (the second level would only add a hop)
+    const uint32_t ng = G < kBarrierTwoLevel ? 1u : kBarrierGroups;
     const uint32_t g = blockIdx.x % ng;
     const uint32_t n = (G - g + ng - 1) / ng;  // workgroups g, g + ng, ... below G
     const unsigned long long v =
         __hip_atomic_fetch_add(grid_group_count(bar, g), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     t.target = (uint32_t)(v / n) + 1u;
     if (v % n == n - 1) {
-      const unsigned long long u =
-          __hip_atomic_fetch_add(grid_coherent_count(bar), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (u % ng == ng - 1) __hip_atomic_store(bar + 96, t.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (ng == 1) {
+        __hip_atomic_store(bar + 96, t.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const unsigned long long u =
+            __hip_atomic_fetch_add(grid_coherent_count(bar), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (u % ng == ng - 1) __hip_atomic_store(bar + 96, t.target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
   }
   return t;
